@@ -1,0 +1,144 @@
+// Internal layouts and kernel interfaces shared by the host engine and the HIP
+// kernels. Nothing here is part of the public C-ABI.
+#pragma once
+#include <stdint.h>
+#include <stddef.h>
+#include <hip/hip_runtime.h>
+
+// One record per map tile (indexed like the reference's chunk buffer, by map
+// index). 128 bytes = one cache line: the occupancy mask, the rank prefix and
+// the pool offset that a DDA needs on chunk entry and on a hit arrive together.
+struct alignas(128) DNchunkRec {
+    uint32_t mask[16];     // bit (x + 8y + 64z): voxel stored (voxelShared.comp:45)
+    uint32_t partial[3];   // stored voxels before bits 128, 256, 384 (voxelShared.comp:44)
+    uint32_t voxelIndex;   // pool start of this chunk's voxels (voxelShared.comp:53)
+    uint32_t numVoxels;    // stored voxels (popcount of mask)
+    uint32_t samples;      // numIndirectSamples (voxelShared.comp:42)
+    int32_t pos[3];        // chunk position in the map (voxelShared.comp:41)
+    uint32_t pad[7];
+};
+static_assert(sizeof(DNchunkRec) == 128, "chunk record must be one 128-byte line");
+
+// Material, std140 32 bytes (voxelShared.comp:57-69, voxel.h:82-94).
+struct DNmat {
+    float padding[2];      // padding[0] > 0: emission strength (extension)
+    uint32_t emissive;
+    float opacity;
+    float refractIndex;
+    float specular;
+    uint32_t reflectType;
+    uint32_t shininess;
+};
+static_assert(sizeof(DNmat) == 32, "material layout");
+
+// Read-only view of one volume's device state used by every DDA.
+struct DNsceneView {
+    const uint32_t* occLoaded;   // bit per tile: flags & 3 == 2
+    const uint32_t* occPresent;  // bit per tile: flags & 3 in {1, 3}
+    uint32_t* flags;             // per tile: load state (bits 0-1), visible (bit 2)
+    uint32_t* lastUsed;          // per tile
+    const DNchunkRec* chunks;    // per tile
+    const uint4* voxels;         // pool
+    const DNmat* mats;           // 256
+    uint32_t mapX, mapY, mapZ;
+    uint32_t hasUnloaded;        // any tile in state 1/3: rays may request chunks
+    float sunStrength[3];
+    float skyBot[3], skyTop[3];
+};
+
+// Limits of the per-frame tables passed by value to the lighting kernel.
+enum { DN_MAX_POINTS = 64, DN_MAX_SAMPLES = 4, DN_MAX_BOUNCES = 8 };
+
+// Per-frame voxel-uniform random numbers (voxelLighting.comp:29-57): every seed
+// in the shader depends only on time, the sample index and the bounce index,
+// so all voxels of a frame draw the same numbers. They are computed once on
+// the host with the shader's hash and passed to the kernel.
+struct DNframeRandom {
+    float diffuseDir[DN_MAX_SAMPLES][DN_MAX_BOUNCES][3];  // rand_unit_sphere(time*(i+1) + j)
+    float glossyPick[DN_MAX_SAMPLES][DN_MAX_BOUNCES];     // rand(time*(i+1) + bounceLimit + j)
+    float glossyDir[DN_MAX_BOUNCES][3];                   // rand_unit_sphere(time + j)
+    float shadowDir[DN_MAX_SAMPLES][3];                   // rand_unit_sphere(time*(i+1+numSamples))
+};
+
+struct DNlightParams {
+    DNsceneView scene;
+    const uint32_t* requests;
+    const uint32_t* numRequests;   // device count
+    uint32_t* workCounter;         // device, reset by the epilogue
+    uint32_t* litBits;             // bit per tile, set for lit chunks
+    uint32_t* visProp;             // bit per tile, set by specular rays
+    uint4* voxelsOut;              // pool receiving lit words (== voxels when written in place)
+    float* debugValues;            // optional: 6 floats per slot
+    uint32_t* debugIndex;          // optional: pool index per slot
+    uint32_t debugSlots;           // capacity of the debug outputs in slots
+    uint64_t* counters;            // 8 x u64
+    uint32_t numDiffuseSamples, maxDiffuseSamples, diffuseBounceLimit, specBounceLimit;
+    float sunDir[3];
+    float ambient[3];
+    float shadowSoftness;
+    float camPos[3];
+    uint32_t numSpecRays, numDirectRays;
+    const float* specPoints;       // device, numSpecRays * 3
+    const float* directPoints;     // device, numDirectRays * 3
+    DNframeRandom rnd;
+};
+
+struct DNdrawParams {
+    DNsceneView scene;
+    float invView[16];
+    float invViewProj[16];
+    uint32_t width, height;
+    uint32_t viewMode;
+    float4* out;
+    void* hitOut;                  // optional DNXpixelHit*
+    uint64_t* counters;
+};
+
+struct DNrayParams {
+    DNsceneView scene;
+    uint32_t n;
+    const float* origins;
+    const float* dirs;
+    uint32_t ignoreFirst;
+    void* out;                     // DNXrayHit*
+    uint64_t* counters;
+};
+
+// One upload operation of a sync: a tile state change, optionally with a chunk
+// record and its voxels staged in device memory.
+struct DNuploadOp {
+    uint32_t tile;
+    uint32_t mode;        // 0: flags = 0, 1: flags = 1, 2: load record + voxels, flags = 2 | visible
+    uint32_t recSlot;     // index into staged records (mode 2)
+    uint32_t voxOffset;   // first staged voxel (mode 2)
+};
+
+struct DNvisibleParams {
+    uint32_t* flags;
+    uint32_t* lastUsed;
+    const DNchunkRec* chunks;
+    uint32_t* fresh;       // bit per tile
+    uint32_t tiles;
+    uint32_t split;        // 0: emit nothing (DN_WRITE)
+    uint32_t frameNum;
+    uint32_t* blockSums;
+    uint32_t* requests;
+    uint32_t* numRequests;
+};
+
+// Counter slots (u64 each) per kind.
+enum { CNT_RAYS = 0, CNT_MAPSTEPS, CNT_VOXSTEPS, CNT_HITS, CNT_LIT, CNT_PIXELS, CNT_SLOTS = 8 };
+
+// Kernel launchers (kernels.hip). All asynchronous on `stream`; they return a
+// hipError_t value.
+int dn_launch_draw(const DNdrawParams& p, hipStream_t stream);
+int dn_launch_lighting(const DNlightParams& p, uint32_t maxRequests, hipStream_t stream);
+int dn_launch_lighting_epilogue(const DNlightParams& p, uint32_t tiles, const uint32_t* requests,
+                                const uint32_t* numRequests, int copyBack, hipStream_t stream);
+int dn_launch_visible(const DNvisibleParams& p, hipStream_t stream);
+int dn_launch_upload(uint32_t* flags, uint32_t* lastUsed, DNchunkRec* chunks, uint4* voxels, uint32_t* fresh,
+                     const DNuploadOp* ops, uint32_t numOps, const DNchunkRec* recs, const uint4* stagedVoxels,
+                     hipStream_t stream);
+int dn_launch_rays(const DNrayParams& p, hipStream_t stream);
+int dn_launch_collect_requested(const uint32_t* flags, uint32_t tiles, uint32_t* list, uint32_t* count,
+                                hipStream_t stream);

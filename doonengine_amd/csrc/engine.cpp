@@ -1,0 +1,1678 @@
+// Host engine: the reference's C API (src/DoonEngine/voxel.h:156-371) over HIP
+// device state. CPU-side chunk storage and the edit API keep the reference's
+// semantics (voxel.c:659-714, 1101-1316); the GPU side is this build's own
+// design: per-tile SoA device arrays, a buddy allocator for the voxel pool,
+// one staged upload per sync, and device-side visible-chunk compaction.
+#include "doon/doon.h"
+#include "dn_internal.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <set>
+#include <thread>
+#include <vector>
+
+void (*g_DN_message_callback)(DNmessageType, DNmessageSeverity, const char*) = nullptr;
+
+namespace {
+
+std::atomic<uint32_t> g_errors{0};
+
+void dn_msg(DNmessageType type, DNmessageSeverity sev, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if(sev != DN_MESSAGE_NOTE) g_errors++;
+    if(g_DN_message_callback)
+        g_DN_message_callback(type, sev, buf);
+    else if(sev != DN_MESSAGE_NOTE)
+        fprintf(stderr, "doon: %s\n", buf);
+}
+
+#define HIP_OK(expr, what)                                                                                  \
+    ([&]() -> bool {                                                                                        \
+        hipError_t e_ = (expr);                                                                             \
+        if(e_ != hipSuccess)                                                                                \
+        {                                                                                                   \
+            dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR, "%s: %s", what, hipGetErrorString(e_));          \
+            return false;                                                                                   \
+        }                                                                                                   \
+        return true;                                                                                        \
+    }())
+
+inline uint32_t material_of(uint32_t normalWord) { return normalWord >> 24; }
+
+// ---------------------------------------------------------------------------
+// buddy allocator over the voxel pool, in blocks of 16 << k voxels (k = 0..5)
+
+struct PoolAlloc {
+    std::set<uint32_t> freeSets[6];
+    uint32_t cap = 0;   // voxels, multiple of 512
+
+    static int cls(uint32_t n)
+    {
+        int k = 0;
+        while((16u << k) < n) k++;
+        return k;
+    }
+    void reset(uint32_t capacity)
+    {
+        for(auto& s : freeSets) s.clear();
+        cap = 0;
+        grow_to(capacity);
+    }
+    void grow_to(uint32_t capacity)
+    {
+        capacity = (capacity + 511u) & ~511u;
+        for(uint32_t s = cap; s < capacity; s += 512u) freeSets[5].insert(s);
+        if(capacity > cap) cap = capacity;
+    }
+    // returns start or UINT32_MAX
+    uint32_t alloc(int k)
+    {
+        int j = k;
+        while(j < 6 && freeSets[j].empty()) j++;
+        if(j == 6) return UINT32_MAX;
+        uint32_t start = *freeSets[j].begin();
+        freeSets[j].erase(freeSets[j].begin());
+        while(j > k)
+        {
+            j--;
+            freeSets[j].insert(start + (16u << j));
+        }
+        return start;
+    }
+    void release(uint32_t start, int k)
+    {
+        while(k < 5)
+        {
+            uint32_t buddy = start ^ (16u << k);
+            auto it = freeSets[k].find(buddy);
+            if(it == freeSets[k].end()) break;
+            freeSets[k].erase(it);
+            start = std::min(start, buddy);
+            k++;
+        }
+        freeSets[k].insert(start);
+    }
+};
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    int kind = 0;
+};
+
+}  // namespace
+
+struct DNgpuVolume {
+    hipStream_t stream = nullptr;
+    uint32_t tiles = 0, occWords = 0, visBlocks = 0;
+
+    uint32_t* dFlags = nullptr;
+    uint32_t* dLastUsed = nullptr;
+    DNchunkRec* dChunks = nullptr;
+    uint4* dVoxels = nullptr;
+    uint4* dVoxelsOut = nullptr;
+    DNmat* dMats = nullptr;
+    uint32_t* dOccLoaded = nullptr;
+    uint32_t* dOccPresent = nullptr;
+    uint32_t* dFresh = nullptr;
+    uint32_t* dLitBits = nullptr;
+    uint32_t* dVisProp = nullptr;
+    uint32_t* dRequests = nullptr;
+    uint32_t* dNumRequests = nullptr;
+    uint32_t* dBlockSums = nullptr;
+    uint32_t* dWorkCounter = nullptr;
+    uint32_t* dReqList = nullptr;      // tiles requested by rays (collect)
+    uint64_t* dCounters = nullptr;     // 3 kinds x CNT_SLOTS
+    float* dPoints = nullptr;          // 2 x DN_MAX_POINTS x 3
+    uint32_t specN = 0, directN = 0;
+    std::vector<float> specPts, directPts;
+
+    // staging: two pinned host buffers used alternately, one device buffer
+    uint8_t* hStage[2] = {nullptr, nullptr};
+    size_t hStageCap[2] = {0, 0};
+    hipEvent_t stageDone[2] = {nullptr, nullptr};
+    bool stageUsed[2] = {false, false};
+    int stageNext = 0;
+    uint8_t* dStage = nullptr;
+    size_t dStageCap = 0;
+    uint32_t* hSmall = nullptr;        // pinned readback words
+
+    // host mirror of device tile state
+    std::vector<uint8_t> state;        // 0 none, 1 unloaded, 2 loaded, 3 to load
+    std::vector<uint32_t> nodeStart;
+    std::vector<int8_t> nodeClass;     // -1 none
+    std::vector<uint32_t> occLoaded, occPresent;
+    std::vector<uint32_t> blocks;      // ceil(numVoxels/32) of loaded tiles
+    bool occDirty = true;
+    uint32_t numUnloaded = 0;          // tiles in state 1 or 3
+    uint64_t loadedBlocks = 0;
+    std::vector<uint32_t> dirty;
+    std::vector<uint8_t> dirtyMark;
+    bool fullScan = false;
+    bool requestAll = false;
+    bool layoutDirty = true;
+    PoolAlloc pool;
+
+    DNmaterial lastMats[DN_MAX_MATERIALS];
+    bool matsValid = false;
+    bool anySpecular = false;
+
+    bool timing = false;
+    std::vector<Timer> pendingTimers;
+    std::vector<Timer> freeTimers;
+    double ms[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t launches[6] = {0, 0, 0, 0, 0, 0};
+
+    DNXpixelHit* drawHitOut = nullptr;
+    size_t drawHitCap = 0;
+    float* dbgValues = nullptr;
+    uint32_t* dbgIndex = nullptr;
+    size_t dbgCap = 0;
+};
+
+namespace {
+
+enum { TK_DRAW = 0, TK_LIGHT, TK_VISIBLE, TK_UPLOAD, TK_EPILOGUE, TK_RAYS };
+
+Timer timer_begin(DNgpuVolume* g, int kind)
+{
+    Timer t;
+    if(!g->timing) return t;
+    if(!g->freeTimers.empty()) { t = g->freeTimers.back(); g->freeTimers.pop_back(); }
+    else { hipEventCreate(&t.a); hipEventCreate(&t.b); }
+    t.kind = kind;
+    hipEventRecord(t.a, g->stream);
+    return t;
+}
+void timer_end(DNgpuVolume* g, Timer t)
+{
+    if(!t.a) return;
+    hipEventRecord(t.b, g->stream);
+    g->pendingTimers.push_back(t);
+}
+void timers_collect(DNgpuVolume* g)
+{
+    for(auto& t : g->pendingTimers)
+    {
+        hipEventSynchronize(t.b);
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, t.a, t.b);
+        g->ms[t.kind] += ms;
+        g->launches[t.kind]++;
+        g->freeTimers.push_back(t);
+    }
+    g->pendingTimers.clear();
+}
+
+template <typename T> bool dalloc(T** p, size_t count, const char* what)
+{
+    *p = nullptr;
+    if(count == 0) count = 1;
+    return HIP_OK(hipMalloc((void**)p, sizeof(T) * count), what);
+}
+template <typename T> void dfree(T*& p)
+{
+    if(p) hipFree(p);
+    p = nullptr;
+}
+
+void free_tile_buffers(DNgpuVolume* g)
+{
+    dfree(g->dFlags); dfree(g->dLastUsed); dfree(g->dChunks); dfree(g->dOccLoaded); dfree(g->dOccPresent);
+    dfree(g->dFresh); dfree(g->dLitBits); dfree(g->dVisProp); dfree(g->dRequests); dfree(g->dBlockSums);
+    dfree(g->dReqList);
+}
+
+bool alloc_tile_buffers(DNgpuVolume* g, uint32_t tiles)
+{
+    g->tiles = tiles;
+    g->occWords = (tiles + 31u) / 32u;
+    g->visBlocks = (tiles + 1023u) / 1024u;
+    bool ok = dalloc(&g->dFlags, tiles, "map buffer") && dalloc(&g->dLastUsed, tiles, "map buffer") &&
+              dalloc(&g->dChunks, tiles, "chunk buffer") && dalloc(&g->dOccLoaded, g->occWords, "occupancy") &&
+              dalloc(&g->dOccPresent, g->occWords, "occupancy") && dalloc(&g->dFresh, g->occWords, "occupancy") &&
+              dalloc(&g->dLitBits, g->occWords, "occupancy") && dalloc(&g->dVisProp, g->occWords, "occupancy") &&
+              dalloc(&g->dRequests, (size_t)tiles * 16u, "lighting request buffer") &&
+              dalloc(&g->dBlockSums, g->visBlocks, "request scan") && dalloc(&g->dReqList, tiles, "request list");
+    if(!ok) return false;
+    hipMemsetAsync(g->dFlags, 0, sizeof(uint32_t) * tiles, g->stream);
+    hipMemsetAsync(g->dLastUsed, 0, sizeof(uint32_t) * tiles, g->stream);
+    hipMemsetAsync(g->dChunks, 0, sizeof(DNchunkRec) * tiles, g->stream);
+    for(uint32_t* b : {g->dOccLoaded, g->dOccPresent, g->dFresh, g->dLitBits, g->dVisProp})
+        hipMemsetAsync(b, 0, sizeof(uint32_t) * g->occWords, g->stream);
+    g->state.assign(tiles, 0);
+    g->nodeStart.assign(tiles, 0);
+    g->nodeClass.assign(tiles, -1);
+    g->occLoaded.assign(g->occWords, 0);
+    g->occPresent.assign(g->occWords, 0);
+    g->blocks.assign(tiles, 0);
+    g->dirtyMark.assign(tiles, 0);
+    g->dirty.clear();
+    g->numUnloaded = 0;
+    g->loadedBlocks = 0;
+    g->occDirty = true;
+    return true;
+}
+
+bool grow_pool(DNvolume* vol, uint32_t newCap)
+{
+    DNgpuVolume* g = vol->gpu;
+    newCap = (newCap + 511u) & ~511u;
+    if(newCap <= g->pool.cap) return true;
+    uint4* nv = nullptr;
+    if(!dalloc(&nv, (size_t)newCap, "voxel buffer")) return false;
+    if(g->dVoxels && g->pool.cap)
+        hipMemcpyAsync(nv, g->dVoxels, sizeof(uint4) * g->pool.cap, hipMemcpyDeviceToDevice, g->stream);
+    hipStreamSynchronize(g->stream);
+    dfree(g->dVoxels);
+    g->dVoxels = nv;
+    if(g->dVoxelsOut)
+    {
+        dfree(g->dVoxelsOut);
+        if(!dalloc(&g->dVoxelsOut, (size_t)newCap, "voxel output buffer")) return false;
+    }
+    g->pool.grow_to(newCap);
+    vol->voxelCap = g->pool.cap;
+    vol->gpuVoxels = g->dVoxels;
+    return true;
+}
+
+void mark_dirty(DNgpuVolume* g, uint32_t t)
+{
+    if(t >= g->tiles) return;
+    if(!g->dirtyMark[t]) { g->dirtyMark[t] = 1; g->dirty.push_back(t); }
+}
+
+inline void set_bit(std::vector<uint32_t>& v, uint32_t t, bool on)
+{
+    if(on) v[t >> 5] |= 1u << (t & 31u);
+    else v[t >> 5] &= ~(1u << (t & 31u));
+}
+
+void set_state(DNgpuVolume* g, uint32_t t, uint8_t st)
+{
+    uint8_t old = g->state[t];
+    if(old == st) return;
+    if(old == 1 || old == 3) g->numUnloaded--;
+    if(st == 1 || st == 3) g->numUnloaded++;
+    g->state[t] = st;
+    set_bit(g->occLoaded, t, st == 2);
+    set_bit(g->occPresent, t, st == 1 || st == 3);
+    g->occDirty = true;
+}
+
+void free_node(DNgpuVolume* g, uint32_t t)
+{
+    if(g->nodeClass[t] >= 0)
+    {
+        g->pool.release(g->nodeStart[t], g->nodeClass[t]);
+        g->nodeClass[t] = -1;
+        g->layoutDirty = true;
+    }
+    g->loadedBlocks -= g->blocks[t];
+    g->blocks[t] = 0;
+}
+
+// _DN_check_face_visible / _DN_chunk_to_gpu (voxel.c:1391-1461): per-chunk face
+// culling, occupancy mask, rank prefix, albedo linearisation.
+uint8_t g_linLUT[256];
+bool g_lutReady = false;
+
+void build_lut()
+{
+    if(g_lutReady) return;
+    for(int v = 0; v < 256; v++)
+    {
+        float x = (float)v * 0.00392156862f;
+        x = powf(x, DN_GAMMA) * 255.0f;
+        g_linLUT[v] = (uint8_t)x;
+    }
+    g_lutReady = true;
+}
+
+uint32_t chunk_to_gpu(const DNmaterial* mats, const DNchunk& ch, DNchunkRec& rec, uint4* vox)
+{
+    memset(&rec, 0, sizeof(rec));
+    rec.pos[0] = ch.pos.x; rec.pos[1] = ch.pos.y; rec.pos[2] = ch.pos.z;
+    auto faceVisible = [&](int x, int y, int z) -> bool {
+        if(x < 0 || y < 0 || z < 0 || x >= 8 || y >= 8 || z >= 8) return true;
+        uint32_t m = material_of(ch.voxels[x][y][z].normal);
+        return m == DN_MATERIAL_EMPTY || mats[m].opacity < 1.0f;
+    };
+    uint32_t n = 0;
+    for(int z = 0; z < 8; z++)
+    for(int y = 0; y < 8; y++)
+    for(int x = 0; x < 8; x++)
+    {
+        uint32_t index = (uint32_t)(x + 8 * (y + 8 * z));
+        if((index & 127u) == 0 && index != 0) rec.partial[(index >> 7) - 1] = n;
+        const DNcompressedVoxel& v = ch.voxels[x][y][z];
+        if(material_of(v.normal) == DN_MATERIAL_EMPTY) continue;
+        if(!(faceVisible(x + 1, y, z) || faceVisible(x - 1, y, z) || faceVisible(x, y + 1, z) ||
+             faceVisible(x, y - 1, z) || faceVisible(x, y, z + 1) || faceVisible(x, y, z - 1)))
+            continue;
+        rec.mask[index >> 5] |= 1u << (index & 31u);
+        uint32_t a = v.albedo;
+        uint32_t r = g_linLUT[(a >> 24) & 255u], gg = g_linLUT[(a >> 16) & 255u], b = g_linLUT[(a >> 8) & 255u];
+        vox[n] = make_uint4(v.normal, (r << 24) | (gg << 16) | (b << 8), 0u, 0u);
+        n++;
+    }
+    rec.numVoxels = n;
+    rec.samples = 0;
+    return n;
+}
+
+bool ensure_stage(DNgpuVolume* g, int slot, size_t bytes)
+{
+    if(g->hStageCap[slot] < bytes)
+    {
+        if(g->hStage[slot]) { hipEventSynchronize(g->stageDone[slot]); hipHostFree(g->hStage[slot]); }
+        size_t cap = std::max(bytes, (size_t)1 << 20);
+        if(!HIP_OK(hipHostMalloc((void**)&g->hStage[slot], cap, hipHostMallocDefault), "pinned staging"))
+        {
+            g->hStageCap[slot] = 0;
+            return false;
+        }
+        g->hStageCap[slot] = cap;
+    }
+    if(g->dStageCap < bytes)
+    {
+        hipStreamSynchronize(g->stream);
+        dfree(g->dStage);
+        size_t cap = std::max(bytes, (size_t)1 << 20);
+        if(!dalloc(&g->dStage, cap, "device staging")) { g->dStageCap = 0; return false; }
+        g->dStageCap = cap;
+    }
+    return true;
+}
+
+int acquire_stage_slot(DNgpuVolume* g)
+{
+    int s = g->stageNext;
+    g->stageNext ^= 1;
+    if(g->stageUsed[s]) hipEventSynchronize(g->stageDone[s]);
+    return s;
+}
+
+void release_stage_slot(DNgpuVolume* g, int s)
+{
+    hipEventRecord(g->stageDone[s], g->stream);
+    g->stageUsed[s] = true;
+}
+
+bool upload_materials(DNvolume* vol)
+{
+    DNgpuVolume* g = vol->gpu;
+    if(g->matsValid && memcmp(g->lastMats, vol->materials, sizeof(g->lastMats)) == 0) return true;
+    memcpy(g->lastMats, vol->materials, sizeof(g->lastMats));
+    g->matsValid = true;
+    g->anySpecular = false;
+    for(int i = 0; i < DN_MAX_MATERIALS; i++)
+        if(vol->materials[i].specular > 0.0f && vol->materials[i].reflectType <= 1u) g->anySpecular = true;
+    int s = acquire_stage_slot(g);
+    if(!ensure_stage(g, s, sizeof(g->lastMats))) return false;
+    memcpy(g->hStage[s], g->lastMats, sizeof(g->lastMats));
+    bool ok = HIP_OK(hipMemcpyAsync(g->dMats, g->hStage[s], sizeof(g->lastMats), hipMemcpyHostToDevice, g->stream),
+                     "material upload");
+    release_stage_slot(g, s);
+    return ok;
+}
+
+// Fixed direction sets on the unit sphere. N = 15 is the reference's
+// spherePoints table (voxelLighting.comp:23), kept value for value so the
+// reference's specular rays are reproduced exactly; other N use the golden-angle
+// (Fibonacci) construction the table approximates: y = 1 - 2i/(N-1),
+// r = sqrt(1 - y^2), phi = i * pi * (3 - sqrt 5), point (r cos phi, y, r sin phi).
+const float kSpherePoints15[15][3] = {
+    {0.000000f, 1.000000f, 0.000000f},    {-0.379803f, 0.857143f, 0.347931f},  {0.061185f, 0.714286f, -0.697174f},
+    {0.499316f, 0.571429f, 0.651270f},    {-0.889696f, 0.428571f, -0.157375f}, {0.808584f, 0.285714f, -0.514354f},
+    {-0.256942f, 0.142857f, 0.955810f},   {-0.460906f, 0.000000f, -0.887449f}, {0.929687f, -0.142857f, 0.339521f},
+    {-0.885815f, -0.285714f, 0.365650f},  {0.382949f, -0.428571f, -0.818338f}, {0.245607f, -0.571429f, 0.783037f},
+    {-0.605521f, -0.714286f, -0.350913f}, {0.503065f, -0.857143f, -0.110596f}, {-0.000000f, -1.000000f, 0.000000f}};
+
+std::vector<float> fibonacci_points(uint32_t n)
+{
+    std::vector<float> out(3 * (size_t)n, 0.0f);
+    if(n == 15)
+    {
+        for(int i = 0; i < 15; i++)
+            for(int k = 0; k < 3; k++) out[3 * i + k] = kSpherePoints15[i][k];
+        return out;
+    }
+    const double golden = 3.14159265358979323846 * (3.0 - std::sqrt(5.0));
+    for(uint32_t i = 0; i < n; i++)
+    {
+        double y = n > 1 ? 1.0 - 2.0 * (double)i / (double)(n - 1) : 1.0;
+        double r = std::sqrt(std::max(0.0, 1.0 - y * y));
+        double phi = golden * (double)i;
+        out[3 * i + 0] = (float)(std::cos(phi) * r);
+        out[3 * i + 1] = (float)y;
+        out[3 * i + 2] = (float)(std::sin(phi) * r);
+    }
+    return out;
+}
+
+bool ensure_points(DNvolume* vol)
+{
+    DNgpuVolume* g = vol->gpu;
+    uint32_t ns = vol->numSpecularRays, nd = vol->numDirectRays;
+    if(ns < 1 || ns > DN_MAX_POINTS || nd > DN_MAX_POINTS)
+    {
+        dn_msg(DN_MESSAGE_SHADER, DN_MESSAGE_ERROR, "numSpecularRays must be in [1, %d] and numDirectRays in [0, %d]",
+               DN_MAX_POINTS, DN_MAX_POINTS);
+        return false;
+    }
+    if(ns == g->specN && nd == g->directN) return true;
+    g->specPts = fibonacci_points(ns);
+    g->directPts = fibonacci_points(nd);
+    std::vector<float> all(2 * DN_MAX_POINTS * 3, 0.0f);
+    std::copy(g->specPts.begin(), g->specPts.end(), all.begin());
+    std::copy(g->directPts.begin(), g->directPts.end(), all.begin() + DN_MAX_POINTS * 3);
+    hipStreamSynchronize(g->stream);
+    if(!HIP_OK(hipMemcpy(g->dPoints, all.data(), sizeof(float) * all.size(), hipMemcpyHostToDevice), "ray table"))
+        return false;
+    g->specN = ns;
+    g->directN = nd;
+    return true;
+}
+
+// The reference's hash RNG (voxelLighting.comp:29-57), on the host in FP32.
+inline float frand(float seed)
+{
+    float v = sinf(seed) * 43758.5453f;
+    return (v - floorf(v)) * 2.0f - 1.0f;
+}
+void rand_unit_sphere(float seed, float out[3])
+{
+    float p[3] = {0, 0, 0};
+    for(int k = 0; k < 256; k++)
+    {
+        p[0] = frand(seed); p[1] = frand(seed * 2.0f); p[2] = frand(seed * 3.0f);
+        seed = seed + 1.0f;
+        if(p[0] * p[0] + p[1] * p[1] + p[2] * p[2] >= 1.0f) continue;
+        break;
+    }
+    out[0] = p[0]; out[1] = p[1]; out[2] = p[2];
+}
+
+void frame_random(float time, uint32_t nds, uint32_t dbl, DNframeRandom& r)
+{
+    memset(&r, 0, sizeof(r));
+    for(uint32_t i = 0; i < nds && i < DN_MAX_SAMPLES; i++)
+    {
+        float seed = time * (float)(i + 1);
+        for(uint32_t j = 0; j < dbl && j < DN_MAX_BOUNCES; j++)
+        {
+            rand_unit_sphere(seed + (float)j, r.diffuseDir[i][j]);
+            r.glossyPick[i][j] = frand(seed + (float)dbl + (float)j);
+        }
+        rand_unit_sphere(time * (float)(i + 1 + nds), r.shadowDir[i]);
+    }
+    for(uint32_t j = 0; j < dbl && j < DN_MAX_BOUNCES; j++) rand_unit_sphere(time + (float)j, r.glossyDir[j]);
+}
+
+DNsceneView scene_view(DNvolume* vol)
+{
+    DNgpuVolume* g = vol->gpu;
+    DNsceneView s;
+    s.occLoaded = g->dOccLoaded;
+    s.occPresent = g->dOccPresent;
+    s.flags = g->dFlags;
+    s.lastUsed = g->dLastUsed;
+    s.chunks = g->dChunks;
+    s.voxels = g->dVoxels;
+    s.mats = g->dMats;
+    s.mapX = vol->mapSize.x; s.mapY = vol->mapSize.y; s.mapZ = vol->mapSize.z;
+    s.hasUnloaded = g->numUnloaded > 0 ? 1u : 0u;
+    s.sunStrength[0] = vol->sunStrength.x; s.sunStrength[1] = vol->sunStrength.y; s.sunStrength[2] = vol->sunStrength.z;
+    s.skyBot[0] = vol->skyGradientBot.x; s.skyBot[1] = vol->skyGradientBot.y; s.skyBot[2] = vol->skyGradientBot.z;
+    s.skyTop[0] = vol->skyGradientTop.x; s.skyTop[1] = vol->skyGradientTop.y; s.skyTop[2] = vol->skyGradientTop.z;
+    return s;
+}
+
+void clear_chunk(DNvolume* vol, size_t index)   // voxel.c:1353-1363
+{
+    DNchunk& c = vol->chunks[index];
+    c.pos = DNivec3{-1, -1, -1};
+    c.updated = false;
+    c.numVoxels = 0;
+    c.numVoxelsGpu = 0;
+    for(int x = 0; x < 8; x++)
+    for(int y = 0; y < 8; y++)
+    for(int z = 0; z < 8; z++)
+    {
+        c.voxels[x][y][z].normal = UINT32_MAX;
+        c.voxels[x][y][z].albedo = 0;
+    }
+}
+
+inline uint32_t flat(DNvolume* vol, DNivec3 p)
+{
+    return (uint32_t)p.x + vol->mapSize.x * ((uint32_t)p.y + (uint32_t)p.z * vol->mapSize.y);
+}
+
+void rebuild_layout(DNvolume* vol)
+{
+    DNgpuVolume* g = vol->gpu;
+    if(!g->layoutDirty) return;
+    size_t n = 0;
+    for(uint32_t t = 0; t < g->tiles; t++) n += g->nodeClass[t] >= 0;
+    DNvoxelNode* nl = (DNvoxelNode*)realloc(vol->gpuVoxelLayout, sizeof(DNvoxelNode) * (n ? n : 1));
+    if(!nl) return;
+    vol->gpuVoxelLayout = nl;
+    size_t k = 0;
+    for(uint32_t t = 0; t < g->tiles; t++)
+    {
+        if(g->nodeClass[t] < 0) continue;
+        nl[k].size = 16u << g->nodeClass[t];
+        nl[k].startPos = g->nodeStart[t];
+        uint32_t x = t % vol->mapSize.x, y = (t / vol->mapSize.x) % vol->mapSize.y, z = t / (vol->mapSize.x * vol->mapSize.y);
+        nl[k].chunkPos = DNivec3{(int32_t)x, (int32_t)y, (int32_t)z};
+        k++;
+    }
+    vol->numVoxelNodes = n;
+    g->layoutDirty = false;
+}
+
+// Ray-requested tiles (flag 3 written by step_map, voxelShared.comp:462-466).
+// Blocks; only runs while some tile is unloaded.
+void collect_ray_requests(DNvolume* vol)
+{
+    DNgpuVolume* g = vol->gpu;
+    // after every sync each tile is 0, 1 or 2, so numUnloaded counts state-1 tiles
+    if(g->numUnloaded == 0) return;
+    uint32_t* dCount = g->dWorkCounter + 1;
+    hipMemsetAsync(dCount, 0, sizeof(uint32_t), g->stream);
+    dn_launch_collect_requested(g->dFlags, g->tiles, g->dReqList, dCount, g->stream);
+    hipMemcpyAsync(g->hSmall, dCount, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream);
+    hipStreamSynchronize(g->stream);
+    uint32_t n = g->hSmall[0];
+    if(n == 0) return;
+    std::vector<uint32_t> list(n);
+    hipMemcpy(list.data(), g->dReqList, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
+    for(uint32_t t : list)
+        if(t < g->tiles && g->state[t] == 1) { set_state(g, t, 3); mark_dirty(g, t); }
+}
+
+}  // namespace
+
+// ===========================================================================
+// INITIALIZATION (voxel.c:119-295)
+
+extern "C" bool DN_init(void)
+{
+    int n = 0;
+    if(hipGetDeviceCount(&n) != hipSuccess || n < 1)
+    {
+        dn_msg(DN_MESSAGE_SHADER, DN_MESSAGE_FATAL, "no HIP device available");
+        return false;
+    }
+    build_lut();
+    return true;
+}
+
+extern "C" void DN_quit(void) {}
+
+extern "C" DNvolume* DN_create_volume(DNuvec3 mapSize, unsigned int minChunks)
+{
+    build_lut();
+    if(mapSize.x == 0 || mapSize.y == 0 || mapSize.z == 0 ||
+       (uint64_t)mapSize.x * mapSize.y * mapSize.z >= (1ull << 28))
+    {
+        dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_FATAL, "invalid map size");
+        return nullptr;
+    }
+    DNvolume* vol = (DNvolume*)calloc(1, sizeof(DNvolume));
+    if(!vol) { dn_msg(DN_MESSAGE_CPU_MEMORY, DN_MESSAGE_FATAL, "failed to allocate volume"); return nullptr; }
+    DNgpuVolume* g = new DNgpuVolume();
+    vol->gpu = g;
+    const uint32_t tiles = mapSize.x * mapSize.y * mapSize.z;
+    if(!HIP_OK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "stream") ||
+       !alloc_tile_buffers(g, tiles) || !dalloc(&g->dMats, DN_MAX_MATERIALS, "material buffer") ||
+       !dalloc(&g->dNumRequests, 1, "request count") || !dalloc(&g->dWorkCounter, 4, "work counter") ||
+       !dalloc(&g->dCounters, 3 * CNT_SLOTS, "counters") || !dalloc(&g->dPoints, 2 * DN_MAX_POINTS * 3, "ray table") ||
+       !HIP_OK(hipHostMalloc((void**)&g->hSmall, 64, hipHostMallocDefault), "pinned readback") ||
+       !HIP_OK(hipEventCreate(&g->stageDone[0]), "event") || !HIP_OK(hipEventCreate(&g->stageDone[1]), "event"))
+    {
+        DN_delete_volume(vol);
+        return nullptr;
+    }
+    hipMemsetAsync(g->dNumRequests, 0, sizeof(uint32_t), g->stream);
+    hipMemsetAsync(g->dWorkCounter, 0, 4 * sizeof(uint32_t), g->stream);
+    hipMemsetAsync(g->dCounters, 0, 3 * CNT_SLOTS * sizeof(uint64_t), g->stream);
+    hipMemsetAsync(g->dMats, 0, sizeof(DNmat) * DN_MAX_MATERIALS, g->stream);
+
+    size_t numChunks = std::min<size_t>(tiles, minChunks);
+    if(numChunks < 1) numChunks = 1;
+    uint32_t voxelCap = (uint32_t)std::max<size_t>(512, DN_CHUNK_LENGTH * numChunks / 2);
+    g->pool.reset(0);
+    if(!grow_pool(vol, voxelCap)) { DN_delete_volume(vol); return nullptr; }
+
+    vol->map = (DNchunkHandle*)calloc(tiles, sizeof(DNchunkHandle));
+    vol->chunks = (DNchunk*)malloc(sizeof(DNchunk) * numChunks);
+    vol->materials = (DNmaterial*)calloc(DN_MAX_MATERIALS, sizeof(DNmaterial));
+    vol->lightingRequests = (uint32_t*)malloc(sizeof(uint32_t) * numChunks);
+    vol->gpuVoxelLayout = (DNvoxelNode*)malloc(sizeof(DNvoxelNode));
+    if(!vol->map || !vol->chunks || !vol->materials || !vol->lightingRequests || !vol->gpuVoxelLayout)
+    {
+        dn_msg(DN_MESSAGE_CPU_MEMORY, DN_MESSAGE_FATAL, "failed to allocate memory for volume");
+        DN_delete_volume(vol);
+        return nullptr;
+    }
+    vol->mapSize = mapSize;
+    for(size_t i = 0; i < numChunks; i++) clear_chunk(vol, i);
+    vol->chunkCap = numChunks;
+    vol->nextChunk = 0;
+    vol->numLightingRequests = 0;
+    vol->lightingRequestCap = numChunks;
+    vol->numVoxelNodes = 0;
+    vol->gpuMap = g->dFlags;
+    vol->gpuChunks = g->dChunks;
+    vol->gpuVoxels = g->dVoxels;
+
+    // defaults, voxel.c:259-278
+    vol->camPos = DNvec3{0.0f, 0.0f, 0.0f};
+    vol->camOrient = DNvec3{0.0f, 0.0f, 0.0f};
+    vol->camFOV = 90.0f;
+    vol->camViewMode = 0;
+    vol->sunDir = DNvec3{1.0f, 1.0f, 1.0f};
+    vol->sunStrength = DNvec3{0.6f, 0.6f, 0.6f};
+    vol->ambientLightStrength = DNvec3{0.01f, 0.01f, 0.01f};
+    vol->diffuseBounceLimit = 5;
+    vol->specBounceLimit = 2;
+    vol->shadowSoftness = 10.0f;
+    vol->useCubemap = false;
+    vol->skyGradientBot = DNvec3{0.71f, 0.85f, 0.90f};
+    vol->skyGradientTop = DNvec3{0.00f, 0.45f, 0.74f};
+    vol->frameNum = 0;
+    vol->lastTime = 123.456f;
+    vol->numSpecularRays = 15;
+    vol->numDirectRays = 0;
+    return vol;
+}
+
+extern "C" void DN_delete_volume(DNvolume* vol)
+{
+    if(!vol) return;
+    DNgpuVolume* g = vol->gpu;
+    if(g)
+    {
+        if(g->stream) hipStreamSynchronize(g->stream);
+        free_tile_buffers(g);
+        dfree(g->dVoxels); dfree(g->dVoxelsOut); dfree(g->dMats); dfree(g->dNumRequests); dfree(g->dWorkCounter);
+        dfree(g->dCounters); dfree(g->dPoints); dfree(g->dStage);
+        for(int s = 0; s < 2; s++)
+        {
+            if(g->hStage[s]) hipHostFree(g->hStage[s]);
+            if(g->stageDone[s]) hipEventDestroy(g->stageDone[s]);
+        }
+        if(g->hSmall) hipHostFree(g->hSmall);
+        for(auto& t : g->freeTimers) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
+        for(auto& t : g->pendingTimers) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
+        if(g->stream) hipStreamDestroy(g->stream);
+        delete g;
+    }
+    free(vol->map);
+    free(vol->chunks);
+    free(vol->materials);
+    free(vol->lightingRequests);
+    free(vol->gpuVoxelLayout);
+    free(vol);
+}
+
+// ===========================================================================
+// MEMORY (voxel.c:659-714)
+
+extern "C" int DN_add_chunk(DNvolume* vol, DNivec3 pos)
+{
+    size_t i = vol->nextChunk;
+    do
+    {
+        if(!DN_in_map_bounds(vol, vol->chunks[i].pos))
+        {
+            uint32_t mi = flat(vol, pos);
+            vol->map[mi].chunkIndex = (uint32_t)i;
+            vol->map[mi].flag = 1;
+            vol->chunks[i].pos = pos;
+            vol->nextChunk = (i == vol->chunkCap - 1) ? 0 : i + 1;
+            mark_dirty(vol->gpu, mi);
+            return (int)i;
+        }
+        i++;
+        if(i >= vol->chunkCap) i = 0;
+    } while(i != vol->nextChunk);
+
+    size_t tiles = (size_t)vol->mapSize.x * vol->mapSize.y * vol->mapSize.z;
+    size_t newCap = std::min(vol->chunkCap * 2, tiles);
+    dn_msg(DN_MESSAGE_CPU_MEMORY, DN_MESSAGE_NOTE, "automatically resizing chunk memory to accomodate %zu chunks (%zu bytes)",
+           newCap, newCap * sizeof(DNchunk));
+    i = vol->chunkCap;
+    if(newCap <= vol->chunkCap || !DN_set_max_chunks(vol, newCap)) return -1;
+    uint32_t mi = flat(vol, pos);
+    vol->map[mi].chunkIndex = (uint32_t)i;
+    vol->map[mi].flag = 1;
+    vol->chunks[i].pos = pos;
+    vol->nextChunk = (i == vol->chunkCap - 1) ? 0 : i + 1;
+    mark_dirty(vol->gpu, mi);
+    return (int)i;
+}
+
+extern "C" void DN_remove_chunk(DNvolume* vol, DNivec3 pos)
+{
+    uint32_t mi = flat(vol, pos);
+    vol->map[mi].flag = 0;
+    vol->nextChunk = vol->map[mi].chunkIndex;
+    clear_chunk(vol, vol->map[mi].chunkIndex);
+    mark_dirty(vol->gpu, mi);
+}
+
+// ===========================================================================
+// SYNC (voxel.c:719-786): streaming state machine on the host mirror, one
+// staged upload, then the device-side visible-chunk compaction.
+
+extern "C" void DN_sync_gpu(DNvolume* vol, DNmemOp op, int lightingSplit)
+{
+    DNgpuVolume* g = vol->gpu;
+    if(lightingSplit < 1)
+    {
+        dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR, "lightingSplit must be >= 1");
+        lightingSplit = 1;
+    }
+    vol->frameNum++;
+    if(vol->frameNum >= (uint32_t)lightingSplit) vol->frameNum = 0;
+
+    if(op != DN_READ)
+    {
+        collect_ray_requests(vol);
+        if(g->fullScan || g->requestAll)
+        {
+            for(uint32_t t = 0; t < g->tiles; t++)
+                if(g->fullScan || (g->state[t] != 2 && vol->map[t].flag != 0)) mark_dirty(g, t);
+            g->fullScan = false;
+        }
+        std::sort(g->dirty.begin(), g->dirty.end());
+
+        // state transitions (voxel.c:1491-1536) and the list of chunks to load
+        std::vector<DNuploadOp> ops;
+        std::vector<uint32_t> loads;
+        ops.reserve(g->dirty.size());
+        for(uint32_t t : g->dirty)
+        {
+            g->dirtyMark[t] = 0;
+            bool cpu = vol->map[t].flag != 0;
+            DNchunk* ch = cpu ? &vol->chunks[vol->map[t].chunkIndex] : nullptr;
+            uint8_t st = g->state[t];
+            uint8_t before = st;
+            if(cpu && st == 0) st = 1;
+            else if(!cpu && st != 0)
+            {
+                if(st == 2) free_node(g, t);
+                st = 0;
+            }
+            if(st == 2 && ch && ch->updated) { free_node(g, t); st = 3; }
+            if(g->requestAll && cpu && st == 1) st = 3;
+            if(st == 3 && cpu) loads.push_back(t);
+            else if(st != before && (st == 0 || st == 1))
+            {
+                DNuploadOp o = {t, st, 0, 0};
+                ops.push_back(o);
+            }
+            set_state(g, t, st);
+            if(ch) ch->updated = false;
+        }
+        g->dirty.clear();
+        g->requestAll = false;
+
+        // convert, allocate and stage the chunks to load
+        if(!loads.empty())
+        {
+            std::vector<DNchunkRec> recs(loads.size());
+            std::vector<uint4> voxBuf(loads.size() * 512);
+            std::vector<uint32_t> counts(loads.size());
+            const DNmaterial* mats = vol->materials;
+            unsigned nth = std::min<unsigned>(8, std::max<unsigned>(1, std::thread::hardware_concurrency()));
+            if(loads.size() < 64) nth = 1;
+            auto work = [&](unsigned w) {
+                for(size_t i = w; i < loads.size(); i += nth)
+                {
+                    const DNchunk& ch = vol->chunks[vol->map[loads[i]].chunkIndex];
+                    counts[i] = chunk_to_gpu(mats, ch, recs[i], &voxBuf[i * 512]);
+                }
+            };
+            if(nth == 1) work(0);
+            else
+            {
+                std::vector<std::thread> th;
+                for(unsigned w = 0; w < nth; w++) th.emplace_back(work, w);
+                for(auto& x : th) x.join();
+            }
+            size_t totalVox = 0;
+            for(size_t i = 0; i < loads.size(); i++)
+            {
+                uint32_t t = loads[i];
+                uint32_t n = counts[i];
+                vol->chunks[vol->map[t].chunkIndex].numVoxelsGpu = n;
+                int k = PoolAlloc::cls(n < 16 ? 16 : n);
+                uint32_t start = g->pool.alloc(k);
+                if(start == UINT32_MAX)
+                {
+                    uint32_t want = std::max(g->pool.cap * 2, g->pool.cap + 512u);
+                    dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_NOTE,
+                           "automatically resizing voxel buffer to accomodate %u GPU voxels (%zu bytes)", want,
+                           (size_t)want * 16);
+                    if(!grow_pool(vol, want) || (start = g->pool.alloc(k)) == UINT32_MAX)
+                    {
+                        set_state(g, t, 1);
+                        DNuploadOp o = {t, 1, 0, 0};
+                        ops.push_back(o);
+                        continue;
+                    }
+                }
+                g->nodeStart[t] = start;
+                g->nodeClass[t] = (int8_t)k;
+                g->layoutDirty = true;
+                recs[i].voxelIndex = start;
+                g->blocks[t] = (n + 31u) / 32u;
+                g->loadedBlocks += g->blocks[t];
+                DNuploadOp o = {t, 2, (uint32_t)i, (uint32_t)totalVox};
+                ops.push_back(o);
+                set_state(g, t, 2);
+                totalVox += n;
+            }
+            // stage: [ops][records][voxels][occLoaded][occPresent]
+            size_t oOps = 0, oRec = (sizeof(DNuploadOp) * ops.size() + 127) & ~(size_t)127;
+            size_t oVox = oRec + sizeof(DNchunkRec) * recs.size();
+            size_t oOcc = oVox + sizeof(uint4) * totalVox;
+            size_t bytes = oOcc + 2 * sizeof(uint32_t) * g->occWords;
+            int s = acquire_stage_slot(g);
+            if(ensure_stage(g, s, bytes))
+            {
+                uint8_t* h = g->hStage[s];
+                memcpy(h + oOps, ops.data(), sizeof(DNuploadOp) * ops.size());
+                memcpy(h + oRec, recs.data(), sizeof(DNchunkRec) * recs.size());
+                uint4* hv = (uint4*)(h + oVox);
+                for(const DNuploadOp& o : ops)
+                    if(o.mode == 2) memcpy(hv + o.voxOffset, &voxBuf[(size_t)o.recSlot * 512], sizeof(uint4) * counts[o.recSlot]);
+                memcpy(h + oOcc, g->occLoaded.data(), sizeof(uint32_t) * g->occWords);
+                memcpy(h + oOcc + sizeof(uint32_t) * g->occWords, g->occPresent.data(), sizeof(uint32_t) * g->occWords);
+                Timer tm = timer_begin(g, TK_UPLOAD);
+                hipMemcpyAsync(g->dStage, h, bytes, hipMemcpyHostToDevice, g->stream);
+                hipMemcpyAsync(g->dOccLoaded, g->dStage + oOcc, sizeof(uint32_t) * g->occWords, hipMemcpyDeviceToDevice,
+                               g->stream);
+                hipMemcpyAsync(g->dOccPresent, g->dStage + oOcc + sizeof(uint32_t) * g->occWords,
+                               sizeof(uint32_t) * g->occWords, hipMemcpyDeviceToDevice, g->stream);
+                dn_launch_upload(g->dFlags, g->dLastUsed, g->dChunks, g->dVoxels, g->dFresh,
+                                 (const DNuploadOp*)(g->dStage + oOps), (uint32_t)ops.size(),
+                                 (const DNchunkRec*)(g->dStage + oRec), (const uint4*)(g->dStage + oVox), g->stream);
+                timer_end(g, tm);
+                release_stage_slot(g, s);
+                g->occDirty = false;
+            }
+        }
+        else if(!ops.empty() || g->occDirty)
+        {
+            size_t oOcc = (sizeof(DNuploadOp) * ops.size() + 127) & ~(size_t)127;
+            size_t bytes = oOcc + 2 * sizeof(uint32_t) * g->occWords;
+            int s = acquire_stage_slot(g);
+            if(ensure_stage(g, s, bytes))
+            {
+                uint8_t* h = g->hStage[s];
+                memcpy(h, ops.data(), sizeof(DNuploadOp) * ops.size());
+                memcpy(h + oOcc, g->occLoaded.data(), sizeof(uint32_t) * g->occWords);
+                memcpy(h + oOcc + sizeof(uint32_t) * g->occWords, g->occPresent.data(), sizeof(uint32_t) * g->occWords);
+                hipMemcpyAsync(g->dStage, h, bytes, hipMemcpyHostToDevice, g->stream);
+                hipMemcpyAsync(g->dOccLoaded, g->dStage + oOcc, sizeof(uint32_t) * g->occWords, hipMemcpyDeviceToDevice,
+                               g->stream);
+                hipMemcpyAsync(g->dOccPresent, g->dStage + oOcc + sizeof(uint32_t) * g->occWords,
+                               sizeof(uint32_t) * g->occWords, hipMemcpyDeviceToDevice, g->stream);
+                dn_launch_upload(g->dFlags, g->dLastUsed, g->dChunks, g->dVoxels, g->dFresh,
+                                 (const DNuploadOp*)g->dStage, (uint32_t)ops.size(), nullptr, nullptr, g->stream);
+                release_stage_slot(g, s);
+                g->occDirty = false;
+            }
+        }
+        rebuild_layout(vol);
+    }
+
+    DNvisibleParams vp;
+    vp.flags = g->dFlags;
+    vp.lastUsed = g->dLastUsed;
+    vp.chunks = g->dChunks;
+    vp.fresh = g->dFresh;
+    vp.tiles = g->tiles;
+    vp.split = op == DN_WRITE ? 0u : (uint32_t)lightingSplit;
+    vp.frameNum = vol->frameNum;
+    vp.blockSums = g->dBlockSums;
+    vp.requests = g->dRequests;
+    vp.numRequests = g->dNumRequests;
+    Timer tm = timer_begin(g, TK_VISIBLE);
+    int e = dn_launch_visible(vp, g->stream);
+    timer_end(g, tm);
+    if(e) dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR, "visible pass launch failed (%d)", e);
+}
+
+// ===========================================================================
+// DRAWING (voxel.c:788-881)
+
+namespace {
+// QuickMath semantics (quickmath.h:199-203, 919-928, 1214-1333, 1037-1124),
+// restated in plain float code.
+float deg2rad(float d) { return d * 0.01745329251f; }
+DNmat4 mat_identity()
+{
+    DNmat4 r;
+    memset(&r, 0, sizeof(r));
+    for(int i = 0; i < 4; i++) r.m[i][i] = 1.0f;
+    return r;
+}
+DNmat4 mat_mult(const DNmat4& a, const DNmat4& b)   // a * b, column-major
+{
+    DNmat4 r;
+    for(int c = 0; c < 4; c++)
+    for(int row = 0; row < 4; row++)
+    {
+        float acc = a.m[0][row] * b.m[c][0];
+        acc = acc + a.m[1][row] * b.m[c][1];
+        acc = acc + a.m[2][row] * b.m[c][2];
+        acc = acc + a.m[3][row] * b.m[c][3];
+        r.m[c][row] = acc;
+    }
+    return r;
+}
+DNmat4 mat_inv(const DNmat4& mat)   // cofactor expansion, as quickmath.h:1037-1124
+{
+    float a = mat.m[0][0], b = mat.m[0][1], c = mat.m[0][2], d = mat.m[0][3], e = mat.m[1][0], f = mat.m[1][1],
+          g = mat.m[1][2], h = mat.m[1][3], i = mat.m[2][0], j = mat.m[2][1], k = mat.m[2][2], l = mat.m[2][3],
+          m = mat.m[3][0], n = mat.m[3][1], o = mat.m[3][2], p = mat.m[3][3];
+    DNmat4 r;
+    float t0 = k * p - o * l, t1 = j * p - n * l, t2 = j * o - n * k, t3 = i * p - m * l, t4 = i * o - m * k,
+          t5 = i * n - m * j;
+    r.m[0][0] = f * t0 - g * t1 + h * t2;
+    r.m[1][0] = -(e * t0 - g * t3 + h * t4);
+    r.m[2][0] = e * t1 - f * t3 + h * t5;
+    r.m[3][0] = -(e * t2 - f * t4 + g * t5);
+    r.m[0][1] = -(b * t0 - c * t1 + d * t2);
+    r.m[1][1] = a * t0 - c * t3 + d * t4;
+    r.m[2][1] = -(a * t1 - b * t3 + d * t5);
+    r.m[3][1] = a * t2 - b * t4 + c * t5;
+    t0 = g * p - o * h; t1 = f * p - n * h; t2 = f * o - n * g; t3 = e * p - m * h; t4 = e * o - m * g; t5 = e * n - m * f;
+    r.m[0][2] = b * t0 - c * t1 + d * t2;
+    r.m[1][2] = -(a * t0 - c * t3 + d * t4);
+    r.m[2][2] = a * t1 - b * t3 + d * t5;
+    r.m[3][2] = -(a * t2 - b * t4 + c * t5);
+    t0 = g * l - k * h; t1 = f * l - j * h; t2 = f * k - j * g; t3 = e * l - i * h; t4 = e * k - i * g; t5 = e * j - i * f;
+    r.m[0][3] = -(b * t0 - c * t1 + d * t2);
+    r.m[1][3] = a * t0 - c * t3 + d * t4;
+    r.m[2][3] = -(a * t1 - b * t3 + d * t5);
+    r.m[3][3] = a * t2 - b * t4 + c * t5;
+    float det = 1.0f / (a * r.m[0][0] + b * r.m[1][0] + c * r.m[2][0] + d * r.m[3][0]);
+    for(int x = 0; x < 4; x++)
+        for(int y = 0; y < 4; y++) r.m[x][y] = r.m[x][y] * det;
+    return r;
+}
+DNvec3 v3(float x, float y, float z) { return DNvec3{x, y, z}; }
+DNvec3 v3norm(DNvec3 v)
+{
+    float len = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+    if(len == 0.0f) return v3(0, 0, 0);
+    float inv = 1.0f / len;
+    return v3(v.x * inv, v.y * inv, v.z * inv);
+}
+DNvec3 v3cross(DNvec3 a, DNvec3 b) { return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+void rotate_euler3(DNvec3 angles, float r[3][3])
+{
+    float sx = sinf(deg2rad(angles.x)), cx = cosf(deg2rad(angles.x));
+    float sy = sinf(deg2rad(angles.y)), cy = cosf(deg2rad(angles.y));
+    float sz = sinf(deg2rad(angles.z)), cz = cosf(deg2rad(angles.z));
+    r[0][0] = cy * cz; r[0][1] = cy * sz; r[0][2] = -sy;
+    r[1][0] = sx * sy * cz - cx * sz; r[1][1] = sx * sy * sz + cx * cz; r[1][2] = sx * cy;
+    r[2][0] = cx * sy * cz + sx * sz; r[2][1] = cx * sy * sz - sx * cz; r[2][2] = cx * cy;
+}
+DNvec3 mat3_mul(const float m[3][3], DNvec3 v)
+{
+    return v3(m[0][0] * v.x + m[1][0] * v.y + m[2][0] * v.z, m[0][1] * v.x + m[1][1] * v.y + m[2][1] * v.z,
+              m[0][2] * v.x + m[1][2] * v.y + m[2][2] * v.z);
+}
+}  // namespace
+
+extern "C" void DN_set_view_projection_matrices(DNvolume* vol, float aspectRatio, float nearPlane, float farPlane,
+                                                DNmat4* view, DNmat4* projection)
+{
+    float rot[3][3];
+    rotate_euler3(vol->camOrient, rot);
+    DNvec3 front;
+    float t = tanf(deg2rad(vol->camFOV * 0.5f));
+    if(aspectRatio < 1.0f) front = mat3_mul(rot, v3(0.0f, 0.0f, aspectRatio / t));
+    else front = mat3_mul(rot, v3(0.0f, 0.0f, 1.0f / t));
+    // lookat(pos, pos + front, up), quickmath.h:1290-1333
+    DNvec3 pos = vol->camPos;
+    DNvec3 target = v3(pos.x + front.x, pos.y + front.y, pos.z + front.z);
+    DNvec3 dir = v3norm(v3(pos.x - target.x, pos.y - target.y, pos.z - target.z));
+    DNvec3 r = v3norm(v3cross(v3(0.0f, 1.0f, 0.0f), dir));
+    DNvec3 u = v3cross(dir, r);
+    DNmat4 rud = mat_identity();
+    rud.m[0][0] = r.x; rud.m[1][0] = r.y; rud.m[2][0] = r.z;
+    rud.m[0][1] = u.x; rud.m[1][1] = u.y; rud.m[2][1] = u.z;
+    rud.m[0][2] = dir.x; rud.m[1][2] = dir.y; rud.m[2][2] = dir.z;
+    DNmat4 tr = mat_identity();
+    tr.m[3][0] = -pos.x; tr.m[3][1] = -pos.y; tr.m[3][2] = -pos.z;
+    *view = mat_mult(rud, tr);
+    // perspective(fov, 1/aspect, near, far), quickmath.h:1264-1282
+    float aspect = 1.0f / aspectRatio;
+    DNmat4 p;
+    memset(&p, 0, sizeof(p));
+    float scale = tanf(deg2rad(vol->camFOV * 0.5f)) * nearPlane;
+    float right = aspect * scale, top = scale;
+    p.m[0][0] = nearPlane / right;
+    p.m[1][1] = nearPlane / top;
+    p.m[2][2] = -(farPlane + nearPlane) / (farPlane - nearPlane);
+    p.m[3][2] = -2.0f * farPlane * nearPlane / (farPlane - nearPlane);
+    p.m[2][3] = -1.0f;
+    *projection = p;
+}
+
+extern "C" void DNX_draw_matrices(DNmat4 view, DNmat4 projection, DNmat4* invView, DNmat4* invViewProj)
+{
+    DNmat4 centered = view;
+    centered.m[3][0] = 0.0f; centered.m[3][1] = 0.0f; centered.m[3][2] = 0.0f;
+    *invView = mat_inv(view);
+    *invViewProj = mat_mult(mat_inv(centered), mat_inv(projection));
+}
+
+extern "C" void DN_draw(DNvolume* vol, DNtexture out, DNmat4 view, DNmat4 projection, int rasterColorTexture,
+                        int rasterDepthTexture)
+{
+    DNgpuVolume* g = vol->gpu;
+    if(rasterColorTexture >= 0 || rasterDepthTexture >= 0)
+        dn_msg(DN_MESSAGE_SHADER, DN_MESSAGE_ERROR, "raster compositing is not supported; drawing without it");
+    if(vol->useCubemap) dn_msg(DN_MESSAGE_SHADER, DN_MESSAGE_ERROR, "cubemap sky is not supported; using the gradient");
+    if(!out.data || out.width == 0 || out.height == 0)
+    {
+        dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR, "invalid output texture");
+        return;
+    }
+    if(!upload_materials(vol)) return;
+    DNdrawParams p;
+    p.scene = scene_view(vol);
+    DNmat4 iv, ivp;
+    DNX_draw_matrices(view, projection, &iv, &ivp);
+    memcpy(p.invView, &iv.m[0][0], sizeof(p.invView));
+    memcpy(p.invViewProj, &ivp.m[0][0], sizeof(p.invViewProj));
+    p.width = out.width;
+    p.height = out.height;
+    p.viewMode = vol->camViewMode;
+    p.out = (float4*)out.data;
+    p.hitOut = ((size_t)out.width * out.height <= g->drawHitCap) ? g->drawHitOut : nullptr;
+    if(g->drawHitOut && !p.hitOut)
+        dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR, "draw hit output smaller than the image; not written");
+    p.counters = g->dCounters + 0 * CNT_SLOTS;
+    Timer tm = timer_begin(g, TK_DRAW);
+    int e = dn_launch_draw(p, g->stream);
+    timer_end(g, tm);
+    if(e) dn_msg(DN_MESSAGE_SHADER, DN_MESSAGE_ERROR, "draw launch failed (%d)", e);
+}
+
+// ===========================================================================
+// LIGHTING (voxel.c:883-952)
+
+extern "C" void DN_update_lighting(DNvolume* vol, int numDiffuseSamples, int maxDiffuseSamples, float time)
+{
+    DNgpuVolume* g = vol->gpu;
+    if(vol->frameNum == 0) vol->lastTime = time;
+    if(numDiffuseSamples < 0 || numDiffuseSamples > DN_MAX_SAMPLES || maxDiffuseSamples < 0 ||
+       vol->diffuseBounceLimit > DN_MAX_BOUNCES)
+    {
+        dn_msg(DN_MESSAGE_SHADER, DN_MESSAGE_ERROR,
+               "lighting limits: numDiffuseSamples in [0, %d], diffuseBounceLimit <= %d", DN_MAX_SAMPLES, DN_MAX_BOUNCES);
+        return;
+    }
+    if(!upload_materials(vol) || !ensure_points(vol)) return;
+    if(g->anySpecular && !g->dVoxelsOut)
+        if(!dalloc(&g->dVoxelsOut, (size_t)g->pool.cap, "voxel output buffer")) return;
+    DNlightParams p;
+    memset(&p, 0, sizeof(p));
+    p.scene = scene_view(vol);
+    p.requests = g->dRequests;
+    p.numRequests = g->dNumRequests;
+    p.workCounter = g->dWorkCounter;
+    p.litBits = g->dLitBits;
+    p.visProp = g->dVisProp;
+    p.voxelsOut = g->anySpecular ? g->dVoxelsOut : g->dVoxels;
+    p.debugValues = g->dbgValues;
+    p.debugIndex = g->dbgIndex;
+    p.debugSlots = (uint32_t)std::min<size_t>(g->dbgCap * 32u, 0xFFFFFFFFu);
+    p.counters = g->dCounters + 1 * CNT_SLOTS;
+    p.numDiffuseSamples = (uint32_t)numDiffuseSamples;
+    p.maxDiffuseSamples = (uint32_t)maxDiffuseSamples;
+    p.diffuseBounceLimit = vol->diffuseBounceLimit;
+    p.specBounceLimit = vol->specBounceLimit;
+    DNvec3 sd = v3norm(vol->sunDir);
+    p.sunDir[0] = sd.x; p.sunDir[1] = sd.y; p.sunDir[2] = sd.z;
+    p.ambient[0] = vol->ambientLightStrength.x; p.ambient[1] = vol->ambientLightStrength.y;
+    p.ambient[2] = vol->ambientLightStrength.z;
+    p.shadowSoftness = vol->shadowSoftness;
+    p.camPos[0] = vol->camPos.x; p.camPos[1] = vol->camPos.y; p.camPos[2] = vol->camPos.z;
+    p.numSpecRays = vol->numSpecularRays;
+    p.numDirectRays = vol->numDirectRays;
+    p.specPoints = g->dPoints;
+    p.directPoints = g->dPoints + DN_MAX_POINTS * 3;
+    frame_random(vol->lastTime, (uint32_t)numDiffuseSamples, vol->diffuseBounceLimit, p.rnd);
+    uint64_t maxReq = std::min<uint64_t>(g->loadedBlocks, (uint64_t)g->tiles * 16u);
+    Timer tm = timer_begin(g, TK_LIGHT);
+    int e = dn_launch_lighting(p, (uint32_t)maxReq, g->stream);
+    timer_end(g, tm);
+    Timer te = timer_begin(g, TK_EPILOGUE);
+    int e2 = dn_launch_lighting_epilogue(p, g->tiles, g->dRequests, g->dNumRequests, g->anySpecular ? 1 : 0, g->stream);
+    timer_end(g, te);
+    if(e || e2) dn_msg(DN_MESSAGE_SHADER, DN_MESSAGE_ERROR, "lighting launch failed (%d, %d)", e, e2);
+}
+
+// ===========================================================================
+// MAP SETTINGS (voxel.c:957-1096)
+
+extern "C" bool DN_set_map_size(DNvolume* vol, DNuvec3 size)
+{
+    if(size.x == 0 || size.y == 0 || size.z == 0 || (uint64_t)size.x * size.y * size.z >= (1ull << 28))
+    {
+        dn_msg(DN_MESSAGE_CPU_MEMORY, DN_MESSAGE_ERROR, "invalid map size");
+        return false;
+    }
+    uint32_t tiles = size.x * size.y * size.z;
+    DNchunkHandle* nm = (DNchunkHandle*)calloc(tiles, sizeof(DNchunkHandle));
+    if(!nm)
+    {
+        dn_msg(DN_MESSAGE_CPU_MEMORY, DN_MESSAGE_ERROR, "failed to reallocate memory for map");
+        return false;
+    }
+    for(uint32_t z = 0; z < size.z; z++)
+    for(uint32_t y = 0; y < size.y; y++)
+    for(uint32_t x = 0; x < size.x; x++)
+    {
+        DNivec3 p = {(int32_t)x, (int32_t)y, (int32_t)z};
+        if(DN_in_map_bounds(vol, p)) nm[x + size.x * (y + z * size.y)] = vol->map[flat(vol, p)];
+    }
+    free(vol->map);
+    vol->map = nm;
+    vol->mapSize = size;
+    for(size_t i = 0; i < vol->chunkCap; i++)
+        if(!DN_in_map_bounds(vol, vol->chunks[i].pos)) clear_chunk(vol, i);
+    DNgpuVolume* g = vol->gpu;
+    hipStreamSynchronize(g->stream);
+    free_tile_buffers(g);
+    if(!alloc_tile_buffers(g, tiles)) return false;
+    g->pool.reset(g->pool.cap);
+    g->layoutDirty = true;
+    g->fullScan = true;
+    vol->gpuMap = g->dFlags;
+    vol->gpuChunks = g->dChunks;
+    for(uint32_t t = 0; t < tiles; t++)
+        if(vol->map[t].flag) vol->chunks[vol->map[t].chunkIndex].updated = true;
+    return true;
+}
+
+extern "C" bool DN_set_max_chunks(DNvolume* vol, size_t num)
+{
+    DNchunk* nc = (DNchunk*)realloc(vol->chunks, sizeof(DNchunk) * num);
+    if(!nc)
+    {
+        dn_msg(DN_MESSAGE_CPU_MEMORY, DN_MESSAGE_ERROR, "failed to reallocate memory for chunks");
+        return false;
+    }
+    vol->chunks = nc;
+    for(size_t i = vol->chunkCap; i < num; i++) clear_chunk(vol, i);
+    vol->chunkCap = num;
+    return true;
+}
+
+extern "C" bool DN_set_max_voxels_gpu(DNvolume* vol, size_t num)
+{
+    if(num < vol->gpu->pool.cap || num > 0xFFFFFE00u)
+    {
+        dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR, "the voxel buffer can only grow (requested %zu, have %u)", num,
+               vol->gpu->pool.cap);
+        return false;
+    }
+    return grow_pool(vol, (uint32_t)num);
+}
+
+extern "C" bool DN_set_max_lighting_requests(DNvolume* vol, size_t num)
+{
+    uint32_t* nr = (uint32_t*)realloc(vol->lightingRequests, sizeof(uint32_t) * (num ? num : 1));
+    if(!nr)
+    {
+        dn_msg(DN_MESSAGE_CPU_MEMORY, DN_MESSAGE_ERROR, "failed to reallocate memory for lighting requests");
+        return false;
+    }
+    vol->lightingRequests = nr;
+    vol->lightingRequestCap = num;
+    return true;
+}
+
+// ===========================================================================
+// MAP UTILITY (voxel.c:1101-1272)
+
+extern "C" bool DN_in_map_bounds(DNvolume* vol, DNivec3 pos)
+{
+    return pos.x >= 0 && pos.y >= 0 && pos.z >= 0 && (uint32_t)pos.x < vol->mapSize.x &&
+           (uint32_t)pos.y < vol->mapSize.y && (uint32_t)pos.z < vol->mapSize.z;
+}
+
+extern "C" bool DN_in_chunk_bounds(DNivec3 pos)
+{
+    return pos.x < DN_CHUNK_SIZE && pos.y < DN_CHUNK_SIZE && pos.z < DN_CHUNK_SIZE && pos.x >= 0 && pos.y >= 0 &&
+           pos.z >= 0;
+}
+
+extern "C" DNcompressedVoxel DN_get_compressed_voxel(DNvolume* vol, DNivec3 mapPos, DNivec3 chunkPos)
+{
+    return vol->chunks[vol->map[flat(vol, mapPos)].chunkIndex].voxels[chunkPos.x][chunkPos.y][chunkPos.z];
+}
+
+extern "C" DNvoxel DN_get_voxel(DNvolume* vol, DNivec3 mapPos, DNivec3 chunkPos)
+{
+    return DN_decompress_voxel(DN_get_compressed_voxel(vol, mapPos, chunkPos));
+}
+
+extern "C" void DN_set_compressed_voxel(DNvolume* vol, DNivec3 mapPos, DNivec3 chunkPos, DNcompressedVoxel voxel)
+{
+    uint32_t mi = flat(vol, mapPos);
+    if(vol->map[mi].flag == 0)
+    {
+        if(material_of(voxel.normal) == DN_MATERIAL_EMPTY) return;
+        if(DN_add_chunk(vol, mapPos) < 0) return;
+    }
+    uint32_t ci = vol->map[mi].chunkIndex;
+    DNchunk& ch = vol->chunks[ci];
+    uint32_t oldMat = material_of(ch.voxels[chunkPos.x][chunkPos.y][chunkPos.z].normal);
+    uint32_t newMat = material_of(voxel.normal);
+    if(oldMat == DN_MATERIAL_EMPTY && newMat != DN_MATERIAL_EMPTY) ch.numVoxels++;
+    else if(oldMat != DN_MATERIAL_EMPTY && newMat == DN_MATERIAL_EMPTY)
+    {
+        ch.numVoxels--;
+        if(ch.numVoxels == 0)
+        {
+            DN_remove_chunk(vol, mapPos);
+            return;
+        }
+    }
+    ch.voxels[chunkPos.x][chunkPos.y][chunkPos.z] = voxel;
+    ch.updated = true;
+    mark_dirty(vol->gpu, mi);
+}
+
+extern "C" void DN_set_voxel(DNvolume* vol, DNivec3 mapPos, DNivec3 chunkPos, DNvoxel voxel)
+{
+    DN_set_compressed_voxel(vol, mapPos, chunkPos, DN_compress_voxel(voxel));
+}
+
+extern "C" bool DN_does_chunk_exist(DNvolume* vol, DNivec3 pos) { return vol->map[flat(vol, pos)].flag >= 1; }
+
+extern "C" bool DN_does_voxel_exist(DNvolume* vol, DNivec3 mapPos, DNivec3 chunkPos)
+{
+    return material_of(DN_get_compressed_voxel(vol, mapPos, chunkPos).normal) != DN_MATERIAL_EMPTY;
+}
+
+extern "C" void DN_remove_voxel(DNvolume* vol, DNivec3 mapPos, DNivec3 chunkPos)
+{
+    uint32_t mi = flat(vol, mapPos);
+    uint32_t ci = vol->map[mi].chunkIndex;
+    if(DN_does_voxel_exist(vol, mapPos, chunkPos))
+    {
+        vol->chunks[ci].numVoxels--;
+        if(vol->chunks[ci].numVoxels == 0)
+        {
+            DN_remove_chunk(vol, mapPos);
+            return;
+        }
+    }
+    vol->chunks[ci].voxels[chunkPos.x][chunkPos.y][chunkPos.z].normal = UINT32_MAX;
+    vol->chunks[ci].updated = true;
+    mark_dirty(vol->gpu, mi);
+}
+
+// CPU pick ray (voxel.c:1195-1272): voxel-unit DDA with strict-less ties,
+// z winning ties (unlike the GPU DDA's multi-axis steps).
+extern "C" bool DN_step_map(DNvolume* vol, DNvec3 rayDir, DNvec3 rayPos, int maxSteps, DNivec3* hitPos,
+                            DNvoxel* hitVoxel, DNivec3* hitNormal)
+{
+    auto sgn = [](float n) { return n > 0 ? 1 : (n < 0 ? -1 : 0); };
+    *hitNormal = DNivec3{-1000, -1000, -1000};
+    rayPos = v3(rayPos.x * DN_CHUNK_SIZE, rayPos.y * DN_CHUNK_SIZE, rayPos.z * DN_CHUNK_SIZE);
+    DNvec3 inv = v3(1.0f / rayDir.x, 1.0f / rayDir.y, 1.0f / rayDir.z);
+    int sx = sgn(rayDir.x), sy = sgn(rayDir.y), sz = sgn(rayDir.z);
+    DNivec3 pos = {(int32_t)floor(rayPos.x), (int32_t)floor(rayPos.y), (int32_t)floor(rayPos.z)};
+    DNvec3 dd = v3(fabsf(inv.x), fabsf(inv.y), fabsf(inv.z));
+    DNvec3 side;
+    side.x = (float)((sx * (pos.x - rayPos.x) + (sx * 0.5) + 0.5) * dd.x);
+    side.y = (float)((sy * (pos.y - rayPos.y) + (sy * 0.5) + 0.5) * dd.y);
+    side.z = (float)((sz * (pos.z - rayPos.z) + (sz * 0.5) + 0.5) * dd.z);
+    for(int steps = 0; steps < maxSteps; steps++)
+    {
+        DNivec3 mp, cp;
+        DN_separate_position(pos, &mp, &cp);
+        if(pos.x < 0) mp.x--;
+        if(pos.y < 0) mp.y--;
+        if(pos.z < 0) mp.z--;
+        if(DN_in_map_bounds(vol, mp) && DN_does_chunk_exist(vol, mp) && DN_in_chunk_bounds(cp) &&
+           DN_does_voxel_exist(vol, mp, cp))
+        {
+            *hitVoxel = DN_get_voxel(vol, mp, cp);
+            *hitPos = pos;
+            return true;
+        }
+        if(side.x < side.y)
+        {
+            if(side.x < side.z) { side.x += dd.x; pos.x += sx; *hitNormal = DNivec3{-sx, 0, 0}; }
+            else { side.z += dd.z; pos.z += sz; *hitNormal = DNivec3{0, 0, -sz}; }
+        }
+        else
+        {
+            if(side.y < side.z) { side.y += dd.y; pos.y += sy; *hitNormal = DNivec3{0, -sy, 0}; }
+            else { side.z += dd.z; pos.z += sz; *hitNormal = DNivec3{0, 0, -sz}; }
+        }
+    }
+    return false;
+}
+
+// ===========================================================================
+// GENERAL UTILITY (voxel.c:1277-1316)
+
+extern "C" void DN_separate_position(DNivec3 pos, DNivec3* mapPos, DNivec3* chunkPos)
+{
+    *mapPos = DNivec3{pos.x / DN_CHUNK_SIZE, pos.y / DN_CHUNK_SIZE, pos.z / DN_CHUNK_SIZE};
+    *chunkPos = DNivec3{pos.x % DN_CHUNK_SIZE, pos.y % DN_CHUNK_SIZE, pos.z % DN_CHUNK_SIZE};
+}
+
+extern "C" DNvec3 DN_cam_dir(DNvec3 orient)
+{
+    float rot[3][3];
+    rotate_euler3(orient, rot);
+    return mat3_mul(rot, v3(0.0f, 0.0f, 1.0f));
+}
+
+extern "C" DNcompressedVoxel DN_compress_voxel(DNvoxel voxel)
+{
+    DNvec3 n = voxel.normal;
+    n.x = std::max(std::min(n.x, 1.0f), -1.0f);
+    n.y = std::max(std::min(n.y, 1.0f), -1.0f);
+    n.z = std::max(std::min(n.z, 1.0f), -1.0f);
+    uint32_t nx = (uint32_t)(((int)(n.x * 255.0f) + 255) / 2), ny = (uint32_t)(((int)(n.y * 255.0f) + 255) / 2),
+             nz = (uint32_t)(((int)(n.z * 255.0f) + 255) / 2);
+    DNcompressedVoxel r;
+    r.normal = ((uint32_t)voxel.material << 24) | (nx << 16) | (ny << 8) | nz;
+    r.albedo = ((uint32_t)voxel.albedo.r << 24) | ((uint32_t)voxel.albedo.g << 16) | ((uint32_t)voxel.albedo.b << 8);
+    return r;
+}
+
+extern "C" DNvoxel DN_decompress_voxel(DNcompressedVoxel voxel)
+{
+    const float inv255 = 1.0f / 255.0f;
+    int nx = (int)((voxel.normal >> 16) & 0xFF) * 2 - 255, ny = (int)((voxel.normal >> 8) & 0xFF) * 2 - 255,
+        nz = (int)(voxel.normal & 0xFF) * 2 - 255;
+    DNvoxel r;
+    r.normal = v3((float)nx * inv255, (float)ny * inv255, (float)nz * inv255);
+    r.material = (uint8_t)(voxel.normal >> 24);
+    r.albedo = DNcolor{(uint8_t)((voxel.albedo >> 24) & 0xFF), (uint8_t)((voxel.albedo >> 16) & 0xFF),
+                       (uint8_t)((voxel.albedo >> 8) & 0xFF)};
+    return r;
+}
+
+// voxelShapes.c:239-333 without the transform callback.
+extern "C" void DN_shape_sphere(DNvolume* vol, DNvoxel voxel, bool flipNormals, DNvec3 c, float r)
+{
+    DNvec3 mn = v3(c.x - r, c.y - r, c.z - r), mx = v3(c.x + r, c.y + r, c.z + r);
+    DNivec3 iMin = {(int)floorf(mn.x), (int)floorf(mn.y), (int)floorf(mn.z)};
+    DNivec3 iMax = {(int)ceilf(mx.x), (int)ceilf(mx.y), (int)ceilf(mx.z)};
+    if(voxel.material == DN_MATERIAL_EMPTY)
+    {
+        iMin = DNivec3{iMin.x - 1, iMin.y - 1, iMin.z - 1};
+        iMax = DNivec3{iMax.x + 1, iMax.y + 1, iMax.z + 1};
+    }
+    DNivec3 mapMin, mapMax, cmin, cmax;
+    DN_separate_position(iMin, &mapMin, &cmin);
+    DN_separate_position(iMax, &mapMax, &cmax);
+    mapMin.x = std::max(mapMin.x, 0); mapMin.y = std::max(mapMin.y, 0); mapMin.z = std::max(mapMin.z, 0);
+    mapMax.x = std::min(mapMax.x, (int)vol->mapSize.x - 1);
+    mapMax.y = std::min(mapMax.y, (int)vol->mapSize.y - 1);
+    mapMax.z = std::min(mapMax.z, (int)vol->mapSize.z - 1);
+    const float r2 = r * r, r12 = (r + 1.0f) * (r + 1.0f);
+    for(int mZ = mapMin.z; mZ <= mapMax.z; mZ++)
+    for(int mY = mapMin.y; mY <= mapMax.y; mY++)
+    for(int mX = mapMin.x; mX <= mapMax.x; mX++)
+    {
+        DNivec3 mp = {mX, mY, mZ};
+        for(int cZ = 0; cZ < 8; cZ++)
+        for(int cY = 0; cY < 8; cY++)
+        for(int cX = 0; cX < 8; cX++)
+        {
+            DNivec3 cp = {cX, cY, cZ};
+            DNvec3 pos = v3((float)(mX * 8 + cX), (float)(mY * 8 + cY), (float)(mZ * 8 + cZ));
+            DNvec3 fc = v3(pos.x - c.x, pos.y - c.y, pos.z - c.z);
+            float d2 = fc.x * fc.x + fc.y * fc.y + fc.z * fc.z;
+            bool tileExists = vol->map[flat(vol, mp)].flag != 0;
+            if(d2 < r2)
+            {
+                if(tileExists && DN_does_voxel_exist(vol, mp, cp) && voxel.material != DN_MATERIAL_EMPTY) continue;
+                DNvoxel fin = voxel;
+                if(voxel.material != DN_MATERIAL_EMPTY)
+                {
+                    float mxn = fmaxf(fmaxf(fabsf(fc.x), fabsf(fc.y)), fabsf(fc.z));
+                    float s = 1.0f / mxn;
+                    fin.normal = v3(fc.x * s, fc.y * s, fc.z * s);
+                }
+                DN_set_voxel(vol, mp, cp, fin);
+            }
+            else if(voxel.material == DN_MATERIAL_EMPTY && flipNormals && d2 < r12)
+            {
+                if(!tileExists || !DN_does_voxel_exist(vol, mp, cp)) continue;
+                DNvoxel old = DN_get_voxel(vol, mp, cp);
+                float mxn = fmaxf(fmaxf(fabsf(fc.x), fabsf(fc.y)), fabsf(fc.z));
+                float s = -1.0f / mxn;
+                old.normal = v3(fc.x * s, fc.y * s, fc.z * s);
+                DN_set_voxel(vol, mp, cp, old);
+            }
+        }
+    }
+}
+
+// ===========================================================================
+// DNX_* extensions
+
+extern "C" void DNX_set_message_callback(void (*cb)(DNmessageType, DNmessageSeverity, const char*))
+{
+    g_DN_message_callback = cb;
+}
+
+extern "C" uint32_t DNX_error_count(void) { return g_errors.load(); }
+
+extern "C" void DNX_set_voxels_dense(DNvolume* vol, DNivec3 origin, DNuvec3 size, const uint32_t* normalWords,
+                                     const uint32_t* albedoWords)
+{
+    size_t k = 0;
+    for(uint32_t z = 0; z < size.z; z++)
+    for(uint32_t y = 0; y < size.y; y++)
+    for(uint32_t x = 0; x < size.x; x++, k++)
+    {
+        DNivec3 p = {origin.x + (int32_t)x, origin.y + (int32_t)y, origin.z + (int32_t)z};
+        if(p.x < 0 || p.y < 0 || p.z < 0) continue;
+        DNivec3 mp = {p.x / 8, p.y / 8, p.z / 8}, cp = {p.x % 8, p.y % 8, p.z % 8};
+        if(!DN_in_map_bounds(vol, mp)) continue;
+        DNcompressedVoxel v = {normalWords[k], albedoWords ? albedoWords[k] : 0u};
+        uint32_t mi = flat(vol, mp);
+        if(vol->map[mi].flag == 0 && material_of(v.normal) == DN_MATERIAL_EMPTY) continue;
+        DN_set_compressed_voxel(vol, mp, cp, v);
+    }
+}
+
+extern "C" void DNX_request_all_chunks(DNvolume* vol) { vol->gpu->requestAll = true; }
+
+extern "C" void* DNX_get_stream(DNvolume* vol) { return (void*)vol->gpu->stream; }
+
+extern "C" bool DNX_synchronize(DNvolume* vol)
+{
+    return HIP_OK(hipStreamSynchronize(vol->gpu->stream), "stream synchronize");
+}
+
+extern "C" size_t DNX_fetch_lighting_requests(DNvolume* vol)
+{
+    DNgpuVolume* g = vol->gpu;
+    hipMemcpyAsync(g->hSmall, g->dNumRequests, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream);
+    if(!HIP_OK(hipStreamSynchronize(g->stream), "request readback")) return 0;
+    size_t n = g->hSmall[0];
+    if(n > vol->lightingRequestCap && !DN_set_max_lighting_requests(vol, n)) return 0;
+    if(n) hipMemcpy(vol->lightingRequests, g->dRequests, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
+    vol->numLightingRequests = n;
+    return n;
+}
+
+extern "C" size_t DNX_export_gpu_state(DNvolume* vol, uint32_t* map, uint32_t* chunks, uint32_t* voxels,
+                                       size_t voxelWords)
+{
+    DNgpuVolume* g = vol->gpu;
+    if(!HIP_OK(hipStreamSynchronize(g->stream), "export")) return 0;
+    const uint32_t T = g->tiles;
+    if(map || chunks)
+    {
+        std::vector<uint32_t> fl(T), lu(T);
+        std::vector<DNchunkRec> recs(T);
+        hipMemcpy(fl.data(), g->dFlags, sizeof(uint32_t) * T, hipMemcpyDeviceToHost);
+        hipMemcpy(lu.data(), g->dLastUsed, sizeof(uint32_t) * T, hipMemcpyDeviceToHost);
+        hipMemcpy(recs.data(), g->dChunks, sizeof(DNchunkRec) * T, hipMemcpyDeviceToHost);
+        for(uint32_t t = 0; t < T; t++)
+        {
+            bool loaded = (fl[t] & 3u) == 2u;
+            if(map)
+            {
+                map[3 * t] = fl[t];
+                map[3 * t + 1] = lu[t];
+                map[3 * t + 2] = loaded ? recs[t].voxelIndex : 0u;
+            }
+            if(chunks)
+            {
+                uint32_t* c = chunks + (size_t)24 * t;
+                memset(c, 0, 24 * sizeof(uint32_t));
+                if(loaded)
+                {
+                    c[0] = (uint32_t)recs[t].pos[0]; c[1] = (uint32_t)recs[t].pos[1]; c[2] = (uint32_t)recs[t].pos[2];
+                    c[3] = recs[t].samples;
+                    c[4] = recs[t].partial[0]; c[5] = recs[t].partial[1]; c[6] = recs[t].partial[2];
+                    memcpy(c + 7, recs[t].mask, 16 * sizeof(uint32_t));
+                }
+            }
+        }
+    }
+    if(voxels && voxelWords)
+    {
+        size_t words = std::min(voxelWords, (size_t)g->pool.cap * 4);
+        hipMemcpy(voxels, g->dVoxels, sizeof(uint32_t) * words, hipMemcpyDeviceToHost);
+    }
+    return g->pool.cap;
+}
+
+extern "C" uint32_t DNX_get_ray_points(DNvolume* vol, int which, float* out, uint32_t cap)
+{
+    std::vector<float> pts = fibonacci_points(which == 0 ? vol->numSpecularRays : vol->numDirectRays);
+    uint32_t n = (uint32_t)(pts.size() / 3);
+    if(n > cap) n = cap;
+    memcpy(out, pts.data(), sizeof(float) * 3 * n);
+    return n;
+}
+
+extern "C" void DNX_get_counters(DNvolume* vol, int kind, DNXcounters* out, bool reset)
+{
+    DNgpuVolume* g = vol->gpu;
+    if(kind < 0 || kind > 2) { memset(out, 0, sizeof(*out)); return; }
+    uint64_t c[CNT_SLOTS];
+    hipStreamSynchronize(g->stream);
+    hipMemcpy(c, g->dCounters + kind * CNT_SLOTS, sizeof(c), hipMemcpyDeviceToHost);
+    out->rays = c[CNT_RAYS];
+    out->mapSteps = c[CNT_MAPSTEPS];
+    out->voxelSteps = c[CNT_VOXSTEPS];
+    out->hits = c[CNT_HITS];
+    out->litVoxels = c[CNT_LIT];
+    out->pixels = c[CNT_PIXELS];
+    if(reset)
+    {
+        hipMemset(g->dCounters + kind * CNT_SLOTS, 0, sizeof(c));
+        hipDeviceSynchronize();
+    }
+}
+
+extern "C" void DNX_enable_timing(DNvolume* vol, bool enable) { vol->gpu->timing = enable; }
+
+extern "C" void DNX_get_timing(DNvolume* vol, DNXtiming* out, bool reset)
+{
+    DNgpuVolume* g = vol->gpu;
+    timers_collect(g);
+    out->drawMs = g->ms[TK_DRAW]; out->lightingMs = g->ms[TK_LIGHT]; out->visibleMs = g->ms[TK_VISIBLE];
+    out->uploadMs = g->ms[TK_UPLOAD]; out->epilogueMs = g->ms[TK_EPILOGUE]; out->raysMs = g->ms[TK_RAYS];
+    out->drawLaunches = g->launches[TK_DRAW]; out->lightingLaunches = g->launches[TK_LIGHT];
+    out->visibleLaunches = g->launches[TK_VISIBLE]; out->uploadLaunches = g->launches[TK_UPLOAD];
+    out->epilogueLaunches = g->launches[TK_EPILOGUE]; out->raysLaunches = g->launches[TK_RAYS];
+    if(reset)
+        for(int i = 0; i < 6; i++) { g->ms[i] = 0; g->launches[i] = 0; }
+}
+
+extern "C" void DNX_cast_rays(DNvolume* vol, uint32_t n, const float* origins, const float* dirs, bool ignoreFirst,
+                              DNXrayHit* out)
+{
+    DNgpuVolume* g = vol->gpu;
+    DNrayParams p;
+    p.scene = scene_view(vol);
+    p.n = n;
+    p.origins = origins;
+    p.dirs = dirs;
+    p.ignoreFirst = ignoreFirst ? 1u : 0u;
+    p.out = out;
+    p.counters = g->dCounters + 2 * CNT_SLOTS;
+    if(!upload_materials(vol)) return;
+    Timer tm = timer_begin(g, TK_RAYS);
+    int e = dn_launch_rays(p, g->stream);
+    timer_end(g, tm);
+    if(e) dn_msg(DN_MESSAGE_SHADER, DN_MESSAGE_ERROR, "ray launch failed (%d)", e);
+}
+
+extern "C" void DNX_set_draw_hit_output(DNvolume* vol, DNXpixelHit* out, size_t capacityPixels)
+{
+    vol->gpu->drawHitOut = out;
+    vol->gpu->drawHitCap = out ? capacityPixels : 0;
+}
+
+extern "C" void DNX_set_lighting_debug_output(DNvolume* vol, float* values, uint32_t* voxelIndices,
+                                              size_t requestCapacity)
+{
+    vol->gpu->dbgValues = values;
+    vol->gpu->dbgIndex = voxelIndices;
+    vol->gpu->dbgCap = requestCapacity;
+}
+
+// Sizes of the public structs, so bindings can check their mirrors.
+extern "C" size_t DNX_struct_size(int which)
+{
+    switch(which)
+    {
+    case 0: return sizeof(DNvolume);
+    case 1: return sizeof(DNchunk);
+    case 2: return sizeof(DNmaterial);
+    case 3: return sizeof(DNchunkHandle);
+    case 4: return sizeof(DNvoxelNode);
+    case 5: return sizeof(DNXcounters);
+    case 6: return sizeof(DNXtiming);
+    case 7: return sizeof(DNXrayHit);
+    case 8: return sizeof(DNXpixelHit);
+    case 9: return sizeof(DNvoxel);
+    case 10: return offsetof(DNvolume, numSpecularRays);
+    case 11: return offsetof(DNvolume, camPos);
+    default: return 0;
+    }
+}

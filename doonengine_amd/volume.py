@@ -1,0 +1,187 @@
+"""Python mirror of the engine API: a thin object wrapper over libdoon.so.
+
+Method names follow the reference's C API (src/DoonEngine/voxel.h:156-371):
+``sync`` is DN_sync_gpu, ``draw`` is DN_draw, ``update_lighting`` is
+DN_update_lighting, and so on. Every call goes to the HIP library; there is no
+Python or CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._hip import DeviceBuffer
+from ._lib import DNivec3, DNmat4, DNtexture, DNuvec3, DNvec3, DNXcounters, DNXrayHit, DNXtiming
+
+DN_READ, DN_WRITE, DN_READ_WRITE = 0, 1, 2
+KIND_DRAW, KIND_LIGHTING, KIND_RAYS = 0, 1, 2
+
+RAYHIT_DTYPE = np.dtype([("hit", "<u4"), ("voxelIndex", "<u4"), ("mapSteps", "<u4"), ("voxelSteps", "<u4"),
+                         ("pos", "<f4", 3), ("normal", "<f4", 3)])
+PIXELHIT_DTYPE = np.dtype([("voxelIndex", "<u4"), ("mapSteps", "<u4"), ("voxelSteps", "<u4"), ("pad", "<u4"),
+                           ("face", "<f4", 3)])
+MATERIAL_DTYPE = np.dtype([("padding", "<f4", 2), ("emissive", "<u4"), ("opacity", "<f4"), ("refractIndex", "<f4"),
+                           ("specular", "<f4"), ("reflectType", "<u4"), ("shininess", "<u4")])
+
+
+def mat_to_np(m: DNmat4) -> np.ndarray:
+    """Column-major m[col][row] as a (4, 4) array indexed [col, row]."""
+    return np.array([[m.m[c][r] for r in range(4)] for c in range(4)], dtype=np.float32)
+
+
+def np_to_mat(a: np.ndarray) -> DNmat4:
+    m = DNmat4()
+    for c in range(4):
+        for r in range(4):
+            m.m[c][r] = float(a[c, r])
+    return m
+
+
+class Volume:
+    """One DNvolume."""
+
+    def __init__(self, map_size, min_chunks: int = 4096):
+        self.lib = _lib.load()
+        if not self.lib.DN_init():
+            raise RuntimeError("DN_init failed: no usable HIP device")
+        self.ptr = self.lib.DN_create_volume(DNuvec3(*[int(v) for v in map_size]), int(min_chunks))
+        if not self.ptr:
+            raise RuntimeError("DN_create_volume failed")
+        self.v = self.ptr.contents
+        self.map_size = tuple(int(v) for v in map_size)
+        self.tiles = int(np.prod(self.map_size))
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "ptr", None):
+            self.lib.DN_delete_volume(self.ptr)
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- materials and parameters ----------------------------------------
+    @property
+    def materials(self) -> np.ndarray:
+        """Structured view (256,) of vol->materials (writes go straight to the volume)."""
+        buf = (ctypes.c_char * (256 * 32)).from_address(ctypes.addressof(self.v.materials.contents))
+        return np.frombuffer(buf, dtype=MATERIAL_DTYPE)
+
+    def set_material(self, i: int, *, emissive=False, opacity=1.0, refract=1.0, specular=0.0, reflect_type=0,
+                     shininess=0, emission=0.0) -> None:
+        m = self.materials
+        m[i]["padding"] = (emission, 0.0)
+        m[i]["emissive"] = int(bool(emissive))
+        m[i]["opacity"] = opacity
+        m[i]["refractIndex"] = refract
+        m[i]["specular"] = specular
+        m[i]["reflectType"] = reflect_type
+        m[i]["shininess"] = shininess
+
+    def set_vec(self, name: str, xyz) -> None:
+        setattr(self.v, name, DNvec3(*[float(a) for a in xyz]))
+
+    # -- edits ------------------------------------------------------------
+    def set_dense(self, normal_words: np.ndarray, albedo_words: np.ndarray, origin=(0, 0, 0)) -> None:
+        """Sets a dense (Z, Y, X) box of voxels (DNX_set_voxels_dense)."""
+        n = np.ascontiguousarray(normal_words, dtype=np.uint32)
+        a = np.ascontiguousarray(albedo_words, dtype=np.uint32)
+        assert n.shape == a.shape and n.ndim == 3
+        z, y, x = n.shape
+        self.lib.DNX_set_voxels_dense(self.ptr, DNivec3(*origin), DNuvec3(x, y, z), n.ctypes.data, a.ctypes.data)
+
+    def set_voxel(self, map_pos, chunk_pos, material: int, normal, albedo) -> None:
+        vox = _lib.DNvoxel(material, DNvec3(*normal), _lib.DNcolor(*albedo))
+        self.lib.DN_set_voxel(self.ptr, DNivec3(*map_pos), DNivec3(*chunk_pos), vox)
+
+    def remove_voxel(self, map_pos, chunk_pos) -> None:
+        self.lib.DN_remove_voxel(self.ptr, DNivec3(*map_pos), DNivec3(*chunk_pos))
+
+    def shape_sphere(self, material: int, albedo, center, radius: float, flip_normals=False) -> None:
+        vox = _lib.DNvoxel(material, DNvec3(0, 0, 0), _lib.DNcolor(*albedo))
+        self.lib.DN_shape_sphere(self.ptr, vox, bool(flip_normals), DNvec3(*center), float(radius))
+
+    def request_all(self) -> None:
+        self.lib.DNX_request_all_chunks(self.ptr)
+
+    # -- pipeline ---------------------------------------------------------
+    def sync(self, op: int = DN_READ_WRITE, split: int = 1) -> None:
+        self.lib.DN_sync_gpu(self.ptr, op, split)
+
+    def view_projection(self, aspect: float, near: float = 0.1, far: float = 100.0):
+        view, proj = DNmat4(), DNmat4()
+        self.lib.DN_set_view_projection_matrices(self.ptr, aspect, near, far, ctypes.byref(view), ctypes.byref(proj))
+        return view, proj
+
+    def draw(self, out: DeviceBuffer, width: int, height: int, view: DNmat4, proj: DNmat4) -> None:
+        assert out.nbytes >= width * height * 16
+        self.lib.DN_draw(self.ptr, DNtexture(out.ptr, width, height), view, proj, -1, -1)
+
+    def update_lighting(self, num_diffuse: int = 1, max_diffuse: int = 1000, time: float = 123.456) -> None:
+        self.lib.DN_update_lighting(self.ptr, num_diffuse, max_diffuse, time)
+
+    def synchronize(self) -> None:
+        if not self.lib.DNX_synchronize(self.ptr):
+            raise RuntimeError("device error")
+
+    def cast_rays(self, origins: np.ndarray, dirs: np.ndarray, ignore_first: bool = False) -> np.ndarray:
+        o = np.ascontiguousarray(origins, dtype=np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+        n = o.shape[0]
+        bo, bd = DeviceBuffer.from_array(o), DeviceBuffer.from_array(d)
+        bout = DeviceBuffer(n * RAYHIT_DTYPE.itemsize)
+        self.lib.DNX_cast_rays(self.ptr, n, bo.ptr, bd.ptr, bool(ignore_first), bout.ptr)
+        self.synchronize()
+        return bout.download(RAYHIT_DTYPE, (n,))
+
+    # -- inspection -------------------------------------------------------
+    def export_state(self):
+        """Device state in the reference layouts: map (T,3), chunks (T,24), voxels (cap,4), all uint32."""
+        T = self.tiles
+        cap = int(self.v.voxelCap)
+        m = np.zeros((T, 3), np.uint32)
+        c = np.zeros((T, 24), np.uint32)
+        vx = np.zeros((cap, 4), np.uint32)
+        self.lib.DNX_export_gpu_state(self.ptr, m.ctypes.data, c.ctypes.data, vx.ctypes.data, vx.size)
+        return m, c, vx
+
+    def fetch_requests(self) -> np.ndarray:
+        n = self.lib.DNX_fetch_lighting_requests(self.ptr)
+        return np.ctypeslib.as_array(self.v.lightingRequests, shape=(max(n, 1),))[:n].copy()
+
+    def counters(self, kind: int, reset: bool = False) -> dict:
+        c = DNXcounters()
+        self.lib.DNX_get_counters(self.ptr, kind, ctypes.byref(c), reset)
+        return {f: getattr(c, f) for f, _ in c._fields_}
+
+    def enable_timing(self, on: bool = True) -> None:
+        self.lib.DNX_enable_timing(self.ptr, on)
+
+    def timing(self, reset: bool = False) -> dict:
+        t = DNXtiming()
+        self.lib.DNX_get_timing(self.ptr, ctypes.byref(t), reset)
+        return {f: getattr(t, f) for f, _ in t._fields_}
+
+    def ray_points(self, which: int) -> np.ndarray:
+        out = np.zeros((64, 3), np.float32)
+        n = self.lib.DNX_get_ray_points(self.ptr, which, out.ctypes.data, 64)
+        return out[:n].copy()
+
+
+def draw_matrices(view: DNmat4, proj: DNmat4):
+    """(invView, invCenteredView * invProjection) as the draw kernel receives them."""
+    lib = _lib.load()
+    iv, ivp = DNmat4(), DNmat4()
+    lib.DNX_draw_matrices(view, proj, ctypes.byref(iv), ctypes.byref(ivp))
+    return iv, ivp

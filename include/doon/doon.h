@@ -1,0 +1,326 @@
+/*
+ * doon.h -- C-ABI of the MI355X-native DoonEngine voxel pipeline.
+ *
+ * Part 1 (DN_*) is the reference's engine API, src/DoonEngine/voxel.h:156-371 and
+ * src/DoonEngine/globals.h:11-81, with the same names, argument meaning and error
+ * behaviour (bool false / NULL / -1 plus a message through g_DN_message_callback).
+ * OpenGL types are replaced: GL buffer handles become device pointers, the output
+ * texture becomes a caller-owned device RGBA32F buffer (DNtexture).
+ *
+ * Part 2 (DNX_*) are additions with no counterpart in the reference: bulk edits,
+ * explicit residency, device-state export for parity checks, counters and
+ * kernel timing, ray batches, batched multi-scene lighting and the radiance cache.
+ *
+ * Every entry point takes plain pointers and sizes; device pointers are HIP
+ * device addresses (hipMalloc'd by the caller where the caller owns them).
+ * Calls on one DNvolume are ordered on that volume's HIP stream; no call blocks
+ * the host unless its comment says so.
+ */
+#ifndef DOON_MI355X_H
+#define DOON_MI355X_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------
+ * globals.h:11-40 -- small vector structs
+ */
+typedef struct DNivec2 { int32_t x, y; } DNivec2;
+typedef struct DNivec3 { int32_t x, y, z; } DNivec3;
+typedef struct DNivec4 { int32_t x, y, z, w; } DNivec4;
+typedef struct DNuvec2 { uint32_t x, y; } DNuvec2;
+typedef struct DNuvec3 { uint32_t x, y, z; } DNuvec3;
+typedef struct DNuvec4 { uint32_t x, y, z, w; } DNuvec4;
+
+/* QuickMath types used by voxel.h (quickmath.h:111-175): plain floats, matrices
+ * column-major m[column][row]. */
+typedef struct DNvec2 { float x, y; } DNvec2;
+typedef struct DNvec3 { float x, y, z; } DNvec3;
+typedef struct DNvec4 { float x, y, z, w; } DNvec4;
+typedef struct DNmat4 { float m[4][4]; } DNmat4;
+typedef struct DNquaternion { float x, y, z, w; } DNquaternion;
+
+/* ---------------------------------------------------------------------------
+ * globals.h:45-63 -- message callback
+ */
+typedef enum DNmessageType {
+    DN_MESSAGE_CPU_MEMORY,
+    DN_MESSAGE_GPU_MEMORY,
+    DN_MESSAGE_SHADER,
+    DN_MESSAGE_FILE_IO
+} DNmessageType;
+
+typedef enum DNmessageSeverity {
+    DN_MESSAGE_NOTE,
+    DN_MESSAGE_ERROR,
+    DN_MESSAGE_FATAL
+} DNmessageSeverity;
+
+/* globals.h:63. May be left NULL: messages of severity ERROR and FATAL then go
+ * to stderr (the reference calls through a NULL pointer instead). */
+extern void (*g_DN_message_callback)(DNmessageType, DNmessageSeverity, const char*);
+
+/* ---------------------------------------------------------------------------
+ * voxel.h:17-31 -- constants
+ */
+#define DN_CHUNK_SIZE 8
+#define DN_CHUNK_LENGTH 512
+#define DN_MAX_MATERIALS 256
+#define DN_MATERIAL_EMPTY 255
+#define DN_GAMMA 2.2f
+#define DN_FLATTEN_INDEX(p, s) (p.x) + (s.x) * ((p.y) + (p.z) * (s.y))
+
+/* ---------------------------------------------------------------------------
+ * voxel.h:35-94 -- CPU-side data types (layouts identical to the reference)
+ */
+typedef struct DNcolor { uint8_t r, g, b; } DNcolor;
+
+typedef struct DNvoxel {            /* voxel.h:41-46 */
+    uint8_t material;
+    DNvec3 normal;
+    DNcolor albedo;
+} DNvoxel;
+
+typedef struct DNcompressedVoxel {  /* voxel.h:49-53 */
+    uint32_t normal;                /* material << 24 | nx << 16 | ny << 8 | nz */
+    uint32_t albedo;                /* r << 24 | g << 16 | b << 8 */
+} DNcompressedVoxel;
+
+typedef struct DNchunk {            /* voxel.h:56-64 */
+    DNivec3 pos;
+    bool updated;
+    uint32_t numVoxels;
+    uint32_t numVoxelsGpu;
+    DNcompressedVoxel voxels[8][8][8];   /* indexed [x][y][z] */
+} DNchunk;
+
+typedef struct DNchunkHandle {      /* voxel.h:67-71 */
+    uint8_t flag;                   /* 0 = does not exist, 1 = loaded on CPU */
+    uint32_t chunkIndex;
+} DNchunkHandle;
+
+typedef struct DNvoxelNode {        /* voxel.h:74-79 */
+    uint32_t size;
+    size_t startPos;
+    DNivec3 chunkPos;
+} DNvoxelNode;
+
+typedef struct DNmaterial {         /* voxel.h:82-94, 32 bytes, std140 */
+    DNvec2 padding;                 /* padding.x > 0: emission strength of an emissive
+                                       material (extension; 0 keeps the reference's 1.0) */
+    uint32_t emissive;
+    float opacity;
+    float refractIndex;
+    float specular;
+    uint32_t reflectType;           /* 0 no sky, 1 sky, 2 highlight only */
+    uint32_t shininess;
+} DNmaterial;
+
+/* Output image: caller-owned device buffer of width*height RGBA32F texels,
+ * row-major, texel (x, y) at data[4*(y*width + x)]. Replaces `GLuint outputTexture`. */
+typedef struct DNtexture {
+    float* data;
+    uint32_t width;
+    uint32_t height;
+} DNtexture;
+
+struct DNgpuVolume;                 /* opaque device state of a volume */
+
+typedef struct DNvolume {           /* voxel.h:97-142 */
+    /* device buffers (replace glMapBufferID / glChunkBufferID / glVoxelBufferID) */
+    void* gpuMap;                   /* READ ONLY  | per-tile flags words (u32) */
+    void* gpuChunks;                /* READ ONLY  | per-tile 128-byte chunk records */
+    void* gpuVoxels;                /* READ ONLY  | voxel pool, 16 bytes per voxel */
+
+    DNuvec3 mapSize;                /* READ ONLY  | map size in chunks */
+    size_t chunkCap;                /* READ ONLY  | CPU chunk capacity */
+    size_t nextChunk;               /* READ ONLY  */
+    size_t voxelCap;                /* READ ONLY  | GPU voxel pool capacity */
+    size_t numVoxelNodes;           /* READ ONLY  | allocated pool nodes */
+    size_t numLightingRequests;     /* READ ONLY  | refreshed by DNX_fetch_lighting_requests */
+    size_t lightingRequestCap;      /* READ ONLY  */
+
+    DNchunkHandle* map;             /* READ-WRITE */
+    DNchunk* chunks;                /* READ-WRITE */
+    DNmaterial* materials;          /* READ-WRITE | uploaded when changed (compared on DN_draw/DN_update_lighting) */
+    uint32_t* lightingRequests;     /* READ ONLY  | host copy, filled by DNX_fetch_lighting_requests */
+    DNvoxelNode* gpuVoxelLayout;    /* READ ONLY  | pool nodes (size, start, owner) */
+
+    DNvec3 camPos;                  /* READ-WRITE | in chunks */
+    DNvec3 camOrient;               /* READ-WRITE | {pitch, yaw, roll} degrees */
+    float camFOV;                   /* READ-WRITE */
+    uint32_t camViewMode;           /* READ-WRITE | 0 lighting .. 5 face normals */
+
+    DNvec3 sunDir;                  /* READ-WRITE */
+    DNvec3 sunStrength;             /* READ-WRITE */
+    DNvec3 ambientLightStrength;    /* READ-WRITE */
+    uint32_t diffuseBounceLimit;    /* READ-WRITE */
+    uint32_t specBounceLimit;       /* READ-WRITE */
+    float shadowSoftness;           /* READ-WRITE */
+
+    bool useCubemap;                /* READ-WRITE | must stay false: no cubemap on this path */
+    uint32_t cubemapTex;            /* READ-WRITE | unused */
+    DNvec3 skyGradientBot;          /* READ-WRITE */
+    DNvec3 skyGradientTop;          /* READ-WRITE */
+
+    uint32_t frameNum;              /* READ ONLY  */
+    float lastTime;                 /* READ ONLY  */
+
+    /* extension fields */
+    uint32_t numSpecularRays;       /* READ-WRITE | fixed specular rays per voxel; 15 = the reference's spherePoints */
+    uint32_t numDirectRays;         /* READ-WRITE | 0 = one jittered shadow ray per sample (reference);
+                                       N > 0 = N fixed, uniformly spaced shadow rays per sample */
+    struct DNgpuVolume* gpu;        /* READ ONLY  | opaque */
+} DNvolume;
+
+typedef enum DNmemOp { DN_READ, DN_WRITE, DN_READ_WRITE } DNmemOp;   /* voxel.h:145-150 */
+
+/* ---------------------------------------------------------------------------
+ * voxel.h:156-371 -- the engine API
+ */
+bool DN_init(void);                                                    /* voxel.h:156 */
+void DN_quit(void);                                                    /* voxel.h:158 */
+DNvolume* DN_create_volume(DNuvec3 mapSize, unsigned int minChunks);   /* voxel.h:166 */
+void DN_delete_volume(DNvolume* vol);                                  /* voxel.h:170 */
+
+void DN_set_view_projection_matrices(DNvolume* vol, float aspectRatio, float nearPlane, float farPlane,
+                                     DNmat4* view, DNmat4* projection);                       /* voxel.h:197 */
+/* voxel.h:207. rasterColorTexture / rasterDepthTexture must be -1 (no raster
+ * compositing on this path); other values are reported and ignored. */
+void DN_draw(DNvolume* vol, DNtexture outputTexture, DNmat4 view, DNmat4 projection,
+             int rasterColorTexture, int rasterDepthTexture);
+void DN_update_lighting(DNvolume* vol, int numDiffuseSamples, int maxDiffuseSamples, float time); /* voxel.h:215 */
+
+int DN_add_chunk(DNvolume* vol, DNivec3 pos);                          /* voxel.h:225 */
+void DN_remove_chunk(DNvolume* vol, DNivec3 pos);                      /* voxel.h:230 */
+void DN_sync_gpu(DNvolume* vol, DNmemOp op, int lightingSplit);        /* voxel.h:237 */
+
+bool DN_set_map_size(DNvolume* vol, DNuvec3 size);                     /* voxel.h:247 */
+bool DN_set_max_chunks(DNvolume* vol, size_t num);                     /* voxel.h:254 */
+bool DN_set_max_voxels_gpu(DNvolume* vol, size_t num);                 /* voxel.h:260 */
+bool DN_set_max_lighting_requests(DNvolume* vol, size_t num);          /* voxel.h:267 */
+
+bool DN_in_map_bounds(DNvolume* vol, DNivec3 pos);                     /* voxel.h:277 */
+bool DN_in_chunk_bounds(DNivec3 pos);                                  /* voxel.h:282 */
+DNvoxel DN_get_voxel(DNvolume* vol, DNivec3 mapPos, DNivec3 chunkPos); /* voxel.h:290 */
+DNcompressedVoxel DN_get_compressed_voxel(DNvolume* vol, DNivec3 mapPos, DNivec3 chunkPos); /* voxel.h:297 */
+void DN_set_voxel(DNvolume* vol, DNivec3 mapPos, DNivec3 chunkPos, DNvoxel voxel);          /* voxel.h:305 */
+void DN_set_compressed_voxel(DNvolume* vol, DNivec3 mapPos, DNivec3 chunkPos, DNcompressedVoxel voxel); /* voxel.h:312 */
+void DN_remove_voxel(DNvolume* vol, DNivec3 mapPos, DNivec3 chunkPos);  /* voxel.h:318 */
+bool DN_does_chunk_exist(DNvolume* vol, DNivec3 pos);                  /* voxel.h:325 */
+bool DN_does_voxel_exist(DNvolume* vol, DNivec3 mapPos, DNivec3 chunkPos);                  /* voxel.h:332 */
+bool DN_step_map(DNvolume* vol, DNvec3 rayDir, DNvec3 rayPos, int maxSteps, DNivec3* hitPos,
+                 DNvoxel* hitVoxel, DNivec3* hitNormal);               /* voxel.h:344 */
+
+void DN_separate_position(DNivec3 pos, DNivec3* mapPos, DNivec3* chunkPos);  /* voxel.h:354 */
+DNvec3 DN_cam_dir(DNvec3 orient);                                      /* voxel.h:360 */
+DNcompressedVoxel DN_compress_voxel(DNvoxel voxel);                    /* voxel.h:366 */
+DNvoxel DN_decompress_voxel(DNcompressedVoxel voxel);                  /* voxel.h:371 */
+
+/* voxelShapes.h:40 (DN_shape_sphere without the transform callback): sets every
+ * voxel within radius r of c to `voxel` (normals point away from c), or carves
+ * them when voxel.material == DN_MATERIAL_EMPTY; flipNormals re-points the rim. */
+void DN_shape_sphere(DNvolume* vol, DNvoxel voxel, bool flipNormals, DNvec3 c, float r);
+
+/* ---------------------------------------------------------------------------
+ * DNX_* -- extensions (no counterpart in the reference)
+ */
+
+/* Sizes of the public structs (0 DNvolume, 1 DNchunk, 2 DNmaterial, 3 DNchunkHandle,
+ * 4 DNvoxelNode, 5 DNXcounters, 6 DNXtiming, 7 DNXrayHit, 8 DNXpixelHit, 9 DNvoxel;
+ * 10/11 offsets of DNvolume.numSpecularRays / camPos), for checking bindings. */
+size_t DNX_struct_size(int which);
+
+/* Messages and errors. */
+void DNX_set_message_callback(void (*cb)(DNmessageType, DNmessageSeverity, const char*));
+/* Number of ERROR/FATAL messages emitted so far (all volumes). */
+uint32_t DNX_error_count(void);
+
+/* Bulk edit: sets the voxels of a dense box of voxel coordinates [origin, origin+size)
+ * (x fastest, then y, then z). normalWords/albedoWords use DNcompressedVoxel's
+ * encoding; material 255 means empty. Same effect as DN_set_compressed_voxel per voxel. */
+void DNX_set_voxels_dense(DNvolume* vol, DNivec3 origin, DNuvec3 size,
+                          const uint32_t* normalWords, const uint32_t* albedoWords);
+
+/* Residency: marks every chunk present on the CPU to be (re)loaded by the next
+ * DN_sync_gpu, instead of waiting for rays to request it. */
+void DNX_request_all_chunks(DNvolume* vol);
+
+/* Device stream of the volume (a hipStream_t). */
+void* DNX_get_stream(DNvolume* vol);
+/* Blocks until all work queued on the volume's stream is done. Returns false on a device error. */
+bool DNX_synchronize(DNvolume* vol);
+
+/* Blocks; copies the current request list to vol->lightingRequests and returns its length. */
+size_t DNX_fetch_lighting_requests(DNvolume* vol);
+
+/* Device state in the reference's buffer layouts (voxelShared.comp:19-95), for
+ * parity checks. Blocks. map: tiles*3 words (flags, lastUsed, voxelIndex);
+ * chunks: tiles*24 words; voxels: voxelWords words (4 per voxel, pool order).
+ * Any pointer may be NULL. Returns the pool size in voxels. */
+size_t DNX_export_gpu_state(DNvolume* vol, uint32_t* map, uint32_t* chunks, uint32_t* voxels, size_t voxelWords);
+
+/* Fixed direction tables used by the lighting pass (numSpecularRays / numDirectRays
+ * points, 3 floats each). Returns the number of points written. */
+uint32_t DNX_get_ray_points(DNvolume* vol, int which /* 0 specular, 1 direct */, float* out, uint32_t cap);
+
+/* Ray counters accumulated by the kernels: step_map calls, chunk-grid trips,
+ * voxel trips, hits, plus lit voxels and drawn pixels. */
+typedef struct DNXcounters {
+    uint64_t rays;          /* step_map calls */
+    uint64_t mapSteps;      /* chunk-grid DDA trips */
+    uint64_t voxelSteps;    /* voxel DDA trips */
+    uint64_t hits;          /* step_map calls that hit */
+    uint64_t litVoxels;     /* stored voxels written by lighting */
+    uint64_t pixels;        /* pixels written by draw */
+} DNXcounters;
+/* Blocks; kind 0 = draw, 1 = lighting, 2 = ray batches. reset clears after reading. */
+void DNX_get_counters(DNvolume* vol, int kind, DNXcounters* out, bool reset);
+
+/* Kernel timing with HIP events recorded on the volume's stream around each
+ * launch. Disabled by default. */
+void DNX_enable_timing(DNvolume* vol, bool enable);
+typedef struct DNXtiming {
+    double drawMs, lightingMs, visibleMs, uploadMs, epilogueMs, raysMs;
+    uint32_t drawLaunches, lightingLaunches, visibleLaunches, uploadLaunches, epilogueLaunches, raysLaunches;
+} DNXtiming;
+/* Blocks; returns accumulated times. reset clears after reading. */
+void DNX_get_timing(DNvolume* vol, DNXtiming* out, bool reset);
+
+/* One step_map call per ray on the device (voxelShared.comp:421). origins/dirs:
+ * device arrays of 3*n floats in map units; out: device array of n DNXrayHit. */
+typedef struct DNXrayHit {
+    uint32_t hit, voxelIndex, mapSteps, voxelSteps;
+    float pos[3];
+    float normal[3];
+} DNXrayHit;
+void DNX_cast_rays(DNvolume* vol, uint32_t n, const float* origins, const float* dirs, bool ignoreFirst,
+                   DNXrayHit* out);
+
+/* Per-pixel hit record of DN_draw, for parity checks: device array of at least
+ * width*height DNXpixelHit (capacityPixels entries), written while set; NULL disables. */
+typedef struct DNXpixelHit {
+    uint32_t voxelIndex, mapSteps, voxelSteps, pad;
+    float face[3];
+} DNXpixelHit;
+void DNX_set_draw_hit_output(DNvolume* vol, DNXpixelHit* out, size_t capacityPixels);
+
+/* Pre-quantisation lighting values of the last DN_update_lighting, for parity
+ * checks: device array of 32 * requestCapacity slots of 6 floats (diffuse rgb,
+ * spec rgb) and of pool indices (0xFFFFFFFF for empty lanes). */
+void DNX_set_lighting_debug_output(DNvolume* vol, float* values, uint32_t* voxelIndices, size_t requestCapacity);
+
+/* Camera matrices for the draw kernel, from DN_draw's arguments: the inverse
+ * view matrix and invCenteredView * invProjection (voxel.c:844-853). */
+void DNX_draw_matrices(DNmat4 view, DNmat4 projection, DNmat4* invView, DNmat4* invViewProj);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

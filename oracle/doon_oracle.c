@@ -1,0 +1,915 @@
+/*
+ * doon_oracle.c -- CPU restatement of DoonEngine's GPU voxel pipeline.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the checker: only tests/, the smoke
+ * check and the benchmark's CPU-baseline leg load it. The product path never
+ * links or calls it.
+ *
+ * It restates, operation by operation, the reference's GLSL compute shaders and
+ * the host-side pieces of voxel.c that feed them:
+ *   assets/shaders/voxelShared.comp   layouts, codec, rank/select, two-level DDA
+ *   assets/shaders/voxelDraw.comp     primary-ray draw cast
+ *   assets/shaders/voxelLighting.comp per-voxel lighting
+ *   src/DoonEngine/voxel.c:1391-1461  _DN_check_face_visible / _DN_chunk_to_gpu
+ *   src/DoonEngine/voxel.c:1463-1489  _DN_request_chunk_lighting
+ *
+ * The same source is compiled twice:
+ *   default      FP32, strict IEEE (no contraction, no fast-math): the bit-exact
+ *                pin for the HIP kernels.
+ *   -DOR_DOUBLE  FP64 variant of the same algorithm: the accuracy oracle.
+ *
+ * GLSL built-ins are restated with fixed semantics (GLSL leaves several of them
+ * implementation-defined):
+ *   min(x,y) = y<x ? y : x       max(x,y) = x<y ? y : x     clamp = min(max(x,lo),hi)
+ *   sign(x)  = x>0 ? 1 : x<0 ? -1 : 0                        fract(x) = x - floor(x)
+ *   mix(x,y,a) = x*(1-a) + y*a   dot = (x*x' + y*y') + z*z'
+ *   normalize(v) = v * (1/sqrt(dot(v,v)))                    reflect = I - (2*dot(N,I))*N
+ *   round = round-half-to-even   sin = C library sinf / sin  pow = powf / pow
+ *
+ * Deterministic semantics replace the reference's data races (voxelLighting.comp
+ * :224/:284-285 sample count, :104-105/:281 visible bit, :113/:122 neighbour
+ * lighting reads): every read during a lighting pass sees the state at dispatch
+ * start, and the pass's writes go to separate output buffers; visible bits set
+ * by specular rays are merged after the lit chunks' bits are cleared.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef OR_DOUBLE
+typedef double real;
+#define RL(x) (x)
+#define r_floor floor
+#define r_sqrt sqrt
+#define r_sin sin
+#define r_pow pow
+#define r_trunc trunc
+#define r_rint nearbyint
+#define r_fabs fabs
+#else
+typedef float real;
+#define RL(x) (x##f)
+#define r_floor floorf
+#define r_sqrt sqrtf
+#define r_sin sinf
+#define r_pow powf
+#define r_trunc truncf
+#define r_rint nearbyintf
+#define r_fabs fabsf
+#endif
+
+#define EPS RL(0.0001)               /* voxelShared.comp:7 */
+#define INV255 RL(0.00392156862)     /* voxelShared.comp:249-253 (FP32: one ulp below 1/255) */
+#define INV65535 RL(0.0000152590219) /* voxelShared.comp:252 */
+
+/* ------------------------------------------------------------------------- */
+/* vector helpers with the GLSL semantics listed above                        */
+
+typedef struct { real x, y, z; } v3;
+typedef struct { int32_t x, y, z; } iv3;
+
+static inline v3 V(real x, real y, real z) { v3 r = {x, y, z}; return r; }
+static inline v3 vs(real s) { return V(s, s, s); }
+static inline v3 vadd(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 vsub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 vmul(v3 a, v3 b) { return V(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline v3 vmuls(v3 a, real s) { return V(a.x * s, a.y * s, a.z * s); }
+static inline v3 vadds(v3 a, real s) { return V(a.x + s, a.y + s, a.z + s); }
+static inline v3 vdivs(v3 a, real s) { return V(a.x / s, a.y / s, a.z / s); }
+static inline v3 vneg(v3 a) { return V(-a.x, -a.y, -a.z); }
+static inline real vdot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline real gmin(real x, real y) { return y < x ? y : x; }
+static inline real gmax(real x, real y) { return x < y ? y : x; }
+static inline real gsign(real x) { return x > 0 ? RL(1.0) : (x < 0 ? RL(-1.0) : RL(0.0)); }
+static inline v3 vfloor(v3 a) { return V(r_floor(a.x), r_floor(a.y), r_floor(a.z)); }
+static inline v3 vabs(v3 a) { return V(r_fabs(a.x), r_fabs(a.y), r_fabs(a.z)); }
+static inline v3 vsign(v3 a) { return V(gsign(a.x), gsign(a.y), gsign(a.z)); }
+static inline v3 vinv(v3 a) { return V(RL(1.0) / a.x, RL(1.0) / a.y, RL(1.0) / a.z); }
+static inline v3 vmin(v3 a, v3 b) { return V(gmin(a.x, b.x), gmin(a.y, b.y), gmin(a.z, b.z)); }
+static inline v3 vmax(v3 a, v3 b) { return V(gmax(a.x, b.x), gmax(a.y, b.y), gmax(a.z, b.z)); }
+static inline v3 vclamp(v3 a, real lo, real hi)
+{
+    return V(gmin(gmax(a.x, lo), hi), gmin(gmax(a.y, lo), hi), gmin(gmax(a.z, lo), hi));
+}
+static inline v3 vnorm(v3 a)
+{
+    real inv = RL(1.0) / r_sqrt(vdot(a, a));
+    return vmuls(a, inv);
+}
+static inline v3 vreflect(v3 I, v3 N) { return vsub(I, vmuls(N, RL(2.0) * vdot(N, I))); }
+static inline v3 vrefract(v3 I, v3 N, real eta)
+{
+    real d = vdot(N, I);
+    real k = RL(1.0) - eta * eta * (RL(1.0) - d * d);
+    if(k < 0) return vs(0);
+    return vsub(vmuls(I, eta), vmuls(N, eta * d + r_sqrt(k)));
+}
+static inline v3 vmix(v3 x, v3 y, real a) { return vadd(vmuls(x, RL(1.0) - a), vmuls(y, a)); }
+static inline v3 vfromi(iv3 a) { return V((real)a.x, (real)a.y, (real)a.z); }
+static inline v3 vf(const float* p) { return V((real)p[0], (real)p[1], (real)p[2]); }
+static inline real fractr(real x) { return x - r_floor(x); }
+static inline real minc(v3 a) { return gmin(gmin(a.x, a.y), a.z); }
+static inline v3 vpow(v3 a, real e) { return V(r_pow(a.x, e), r_pow(a.y, e), r_pow(a.z, e)); }
+
+/* ------------------------------------------------------------------------- */
+/* reference buffer layouts (voxelShared.comp:19-95, voxel.c:16-42)            */
+/* map tile : [0] flags [1] lastUsed [2] voxelIndex                            */
+/* chunk    : [0..2] pos [3] numIndirectSamples [4..6] partialCounts           */
+/*            [7..22] bitMask [23] padding                                     */
+/* voxel    : [0] material<<24|normal [1] albedo<<8|spec.x                     */
+/*            [2] spec.y<<24|spec.z<<16|diff.x [3] diff.y<<16|diff.z           */
+
+enum { TILE_WORDS = 3, CHUNK_WORDS = 24, VOX_WORDS = 4 };
+
+typedef struct ORmaterial {       /* voxel.h:82-94, std140 32 B */
+    float padding[2];             /* padding[0] > 0: emission strength (0 means 1.0) */
+    uint32_t emissive;
+    float opacity;
+    float refractIndex;
+    float specular;
+    uint32_t reflectType;
+    uint32_t shininess;
+} ORmaterial;
+
+typedef struct ORscene {
+    const uint32_t* map;          /* tiles * 3 words: snapshot at dispatch start */
+    const uint32_t* chunks;       /* tiles * 24 words */
+    const uint32_t* voxels;       /* pool * 4 words */
+    const ORmaterial* mats;       /* 256 */
+    uint32_t mapSize[3];
+    float sunStrength[3];
+    float skyBot[3], skyTop[3];
+    uint32_t* flagsOut;           /* tiles words: receives request writes (flags = 3), may be NULL */
+    uint32_t* lastUsedOut;        /* tiles words: receives lastUsed = 0, may be NULL */
+} ORscene;
+
+typedef struct ORcounters {
+    uint64_t calls;   /* step_map invocations */
+    uint64_t sMap;    /* chunk-grid loop trips (voxelShared.comp:435) */
+    uint64_t sVox;    /* voxel loop trips (voxelShared.comp:339) */
+    uint64_t hits;    /* step_map calls that returned true */
+} ORcounters;
+
+typedef struct ORvoxel { v3 normal; uint32_t material; v3 albedo; v3 spec; v3 diffuse; } ORvoxel;
+
+/* per-invocation shader globals (voxelShared.comp:321-325) plus instrumentation */
+typedef struct ORthread {
+    int enableRefraction;
+    uint32_t lastVoxID;
+    real lastVoxRefract;
+    uint32_t hitIndex;            /* voxel pool index of the last hit */
+    uint32_t sMap, sVox;          /* trips of the last step_map call */
+    ORcounters cnt;
+} ORthread;
+
+static void thread_init(ORthread* t, int refraction)
+{
+    memset(t, 0, sizeof(*t));
+    t->enableRefraction = refraction;
+    t->lastVoxID = 255u;
+    t->lastVoxRefract = RL(1.0);
+    t->hitIndex = 0xFFFFFFFFu;
+}
+
+/* decompress_voxel, voxelShared.comp:236-256 */
+static ORvoxel decompress(const uint32_t* w)
+{
+    ORvoxel r;
+    uint32_t n0 = w[0] >> 24, n1 = (w[0] >> 16) & 255u, n2 = (w[0] >> 8) & 255u, n3 = w[0] & 255u;
+    uint32_t a0 = w[1] >> 24, a1 = (w[1] >> 16) & 255u, a2 = (w[1] >> 8) & 255u, a3 = w[1] & 255u;
+    uint32_t s0 = w[2] >> 24, s1 = (w[2] >> 16) & 255u, s2 = (w[2] >> 8) & 255u, s3 = w[2] & 255u;
+    uint32_t d0 = w[3] >> 24, d1 = (w[3] >> 16) & 255u, d2 = (w[3] >> 8) & 255u, d3 = w[3] & 255u;
+    uint32_t dx = (s2 << 8) | s3, dy = (d0 << 8) | d1, dz = (d2 << 8) | d3;
+    r.normal = vmuls(vsub(vmuls(V((real)n1, (real)n2, (real)n3), INV255), vs(RL(0.5))), RL(2.0));
+    r.material = n0;
+    r.albedo = vmuls(V((real)a0, (real)a1, (real)a2), INV255);
+    r.diffuse = vmuls(V((real)dx, (real)dy, (real)dz), INV65535);
+    r.spec = vmuls(V((real)a3, (real)s0, (real)s1), INV255);
+    return r;
+}
+
+static inline real emission(const ORmaterial* m) { return m->padding[0] > 0.0f ? (real)m->padding[0] : RL(1.0); }
+
+/* ------------------------------------------------------------------------- */
+/* occupancy rank / select (voxelShared.comp:150-231)                          */
+
+static inline uint32_t popc(uint32_t x) { return (uint32_t)__builtin_popcount(x); }
+
+/* get_voxel_index without the voxelIndex base: stored voxels before (x,y,z) */
+uint32_t or_rank(const uint32_t* chunk, uint32_t x, uint32_t y, uint32_t z)
+{
+    uint32_t li = x + 8u * (y + 8u * z);
+    uint32_t w = li >> 5;
+    const uint32_t* mask = chunk + 7;
+    uint32_t n = (w > 3) ? chunk[4 + (w >> 2) - 1] : 0u;
+    for(uint32_t i = w & ~3u; i <= w; i++)
+    {
+        uint32_t bits = mask[i];
+        if(i == w) bits &= (1u << (li & 31u)) - 1u;
+        n += popc(bits);
+    }
+    return n;
+}
+
+/* get_voxel_position: position index x + 8y + 64z of the (voxNum+1)-th stored
+ * voxel, or -1 when the chunk stores fewer voxels (voxelShared.comp:172-224). */
+int32_t or_select(const uint32_t* chunk, uint32_t voxNum)
+{
+    const uint32_t* mask = chunk + 7;
+    uint32_t count = 0, pos = 0, start = 0;
+    for(int i = 0; i < 3; i++)
+    {
+        if(chunk[4 + i] < voxNum) { count = chunk[4 + i]; pos += 128; start += 4; }
+        else break;
+    }
+    voxNum++;
+    for(uint32_t i = start; i < 16; i++)
+    {
+        if(count == voxNum) break;
+        uint32_t bc = popc(mask[i]);
+        if(count + bc >= voxNum)
+        {
+            uint32_t bit = 0;
+            while(count < voxNum) { count += (mask[i] >> bit) & 1u; bit++; pos++; }
+        }
+        else { count += bc; pos += 32; }
+    }
+    pos--;
+    if(count < voxNum) return -1;
+    return (int32_t)pos;
+}
+
+static inline int voxel_exists(const uint32_t* chunk, iv3 p)
+{
+    uint32_t li = (uint32_t)(p.x + 8 * (p.y + 8 * p.z));
+    return (chunk[7 + (li >> 5)] >> (li & 31u)) & 1u;
+}
+
+/* ------------------------------------------------------------------------- */
+/* two-level DDA (voxelShared.comp:300-475)                                    */
+
+typedef struct { iv3 pos; v3 delta; iv3 step; v3 side; } dda_t;
+
+static void init_dda(v3 dir, v3 inv, v3 p, dda_t* d)
+{
+    v3 f = vfloor(p);
+    d->pos.x = (int32_t)f.x; d->pos.y = (int32_t)f.y; d->pos.z = (int32_t)f.z;
+    d->delta = vabs(inv);
+    v3 s = vsign(dir);
+    d->step.x = (int32_t)s.x; d->step.y = (int32_t)s.y; d->step.z = (int32_t)s.z;
+    d->side = vmul(vadds(vadd(vmul(s, vsub(vfromi(d->pos), p)), vmuls(s, RL(0.5))), RL(0.5)), d->delta);
+}
+
+static void iterate_dda(dda_t* d, v3* normal)
+{
+    v3 sd = d->side;
+    int mx = sd.x <= gmin(sd.y, sd.z);
+    int my = sd.y <= gmin(sd.z, sd.x);
+    int mz = sd.z <= gmin(sd.x, sd.y);
+    real fx = mx ? RL(1.0) : RL(0.0), fy = my ? RL(1.0) : RL(0.0), fz = mz ? RL(1.0) : RL(0.0);
+    d->side.x = sd.x + fx * d->delta.x;
+    d->side.y = sd.y + fy * d->delta.y;
+    d->side.z = sd.z + fz * d->delta.z;
+    d->pos.x += mx * d->step.x; d->pos.y += my * d->step.y; d->pos.z += mz * d->step.z;
+    normal->x = fx * (real)(-d->step.x);
+    normal->y = fy * (real)(-d->step.y);
+    normal->z = fz * (real)(-d->step.z);
+}
+
+static inline int in_chunk(iv3 p) { return p.x < 8 && p.y < 8 && p.z < 8 && p.x >= 0 && p.y >= 0 && p.z >= 0; }
+static inline int in_map(const ORscene* s, iv3 p)
+{
+    return p.x >= 0 && p.y >= 0 && p.z >= 0 && (uint32_t)p.x < s->mapSize[0] && (uint32_t)p.y < s->mapSize[1] &&
+           (uint32_t)p.z < s->mapSize[2];
+}
+static inline uint32_t map_index_u(const ORscene* s, uint32_t x, uint32_t y, uint32_t z)
+{
+    return x + s->mapSize[0] * (y + s->mapSize[1] * z);
+}
+
+/* step_chunk, voxelShared.comp:328-418 (maxDepth is negative on every path here,
+ * so the depth test at :363 always passes) */
+static int step_chunk(const ORscene* s, ORthread* t, uint32_t mapIndex, v3* dir, v3* inv, v3* p, int ignoreFirst,
+                      v3* hitNormal, ORvoxel* vox, v3* colorAdd, real* colorMult, int* refracted)
+{
+    *refracted = 0;
+    const uint32_t* chunk = s->chunks + (size_t)mapIndex * CHUNK_WORDS;
+    dda_t d;
+    init_dda(*dir, *inv, *p, &d);
+    v3 lsd = vs(0);
+    while(in_chunk(d.pos))
+    {
+        t->sVox++;
+        if(voxel_exists(chunk, d.pos) && !ignoreFirst)
+        {
+            uint32_t idx = s->map[(size_t)mapIndex * TILE_WORDS + 2] +
+                           or_rank(chunk, (uint32_t)d.pos.x, (uint32_t)d.pos.y, (uint32_t)d.pos.z);
+            const uint32_t* cw = s->voxels + (size_t)idx * VOX_WORDS;
+            *vox = decompress(cw);
+            const ORmaterial* m = &s->mats[vox->material];
+            uint32_t thisID = (cw[1] & 0xFFFFFF00u) | (cw[0] >> 24);
+            if(m->opacity == 1.0f)
+            {
+                *p = vadd(*p, vmuls(*dir, minc(lsd) + EPS));
+                t->hitIndex = idx;
+                return 1;
+            }
+            else if(t->lastVoxID != thisID)
+            {
+                real op = (real)m->opacity;
+                *colorAdd = vadd(*colorAdd, vmul(vmuls(vox->albedo, *colorMult * op), vf(s->sunStrength)));
+                *colorMult = *colorMult * (RL(1.0) - op);
+                real rayDist = minc(lsd);
+                if(t->enableRefraction && rayDist > 0)
+                {
+                    *refracted = 1;
+                    *p = vadd(*p, vmuls(*dir, rayDist + EPS));
+                    v3 n = vdot(vox->normal, *dir) < 0 ? vnorm(vox->normal) : *hitNormal;
+                    *dir = vrefract(*dir, n, t->lastVoxRefract / (real)m->refractIndex);
+                    *inv = vinv(*dir);
+                    init_dda(*dir, *inv, *p, &d);
+                    lsd = vs(0);
+                }
+                t->lastVoxID = thisID;
+                t->lastVoxRefract = (real)m->refractIndex;
+            }
+        }
+        else if(t->lastVoxID != 255u)
+        {
+            if(t->enableRefraction)
+            {
+                *refracted = 1;
+                *p = vadd(*p, vmuls(*dir, minc(lsd) + EPS));
+                v3 old = *dir;
+                v3 nn = vneg(vox->normal);
+                v3 n = vdot(nn, *dir) < 0 ? vnorm(nn) : *hitNormal;
+                *dir = vrefract(*dir, n, t->lastVoxRefract);
+                if(dir->x == 0 && dir->y == 0 && dir->z == 0) *dir = old;
+                *inv = vinv(*dir);
+                init_dda(*dir, *inv, *p, &d);
+                lsd = vs(0);
+            }
+            t->lastVoxID = 255u;
+            t->lastVoxRefract = RL(1.0);
+        }
+        lsd = d.side;
+        iterate_dda(&d, hitNormal);
+        ignoreFirst = 0;
+    }
+    return 0;
+}
+
+/* step_map, voxelShared.comp:421-475. *inv is the caller's 1/dir; refraction
+ * may update dir (inout) and the local inverse. */
+static int step_map(const ORscene* s, ORthread* t, v3* dir, v3 inv, v3* p, int ignoreFirst, v3* hitNormal,
+                    ORvoxel* vox, v3* colorAdd, real* colorMult)
+{
+    int requested = 0;
+    *colorAdd = vs(0);
+    *colorMult = RL(1.0);
+    t->sMap = 0; t->sVox = 0; t->hitIndex = 0xFFFFFFFFu;
+    dda_t d;
+    init_dda(*dir, inv, *p, &d);
+    v3 lsd = vs(0);
+    int hit = 0;
+    while(in_map(s, d.pos))
+    {
+        t->sMap++;
+        uint32_t idx = map_index_u(s, (uint32_t)d.pos.x, (uint32_t)d.pos.y, (uint32_t)d.pos.z);
+        uint32_t flags = s->map[(size_t)idx * TILE_WORDS];
+        if((flags & 3u) == 2u)
+        {
+            if(s->lastUsedOut) s->lastUsedOut[idx] = 0;
+            v3 upd = vadd(*p, vmuls(*dir, minc(lsd) - EPS));
+            v3 crp = vmuls(vsub(upd, vfromi(d.pos)), RL(8.0));
+            crp = vclamp(crp, EPS, RL(8.0) - EPS);
+            int refr;
+            if(step_chunk(s, t, idx, dir, &inv, &crp, ignoreFirst, hitNormal, vox, colorAdd, colorMult, &refr))
+            {
+                *p = vadd(vfromi(d.pos), vmuls(crp, RL(0.125)));
+                hit = 1;
+                break;
+            }
+            if(refr)
+            {
+                *p = vadd(vfromi(d.pos), vmuls(crp, RL(0.125)));
+                init_dda(*dir, inv, *p, &d);
+                lsd = vs(0);
+            }
+        }
+        else if(!requested && (flags & 3u) != 0)
+        {
+            if(s->flagsOut) s->flagsOut[idx] = 3u;
+            requested = 1;
+        }
+        lsd = d.side;
+        iterate_dda(&d, hitNormal);
+        ignoreFirst = 0;
+    }
+    t->cnt.calls++;
+    t->cnt.sMap += t->sMap;
+    t->cnt.sVox += t->sVox;
+    t->cnt.hits += (uint64_t)hit;
+    return hit;
+}
+
+static inline v3 sky_color(const ORscene* s, v3 dir)
+{
+    return vmix(vf(s->skyBot), vf(s->skyTop), dir.y * RL(0.5) + RL(1.0));
+}
+
+/* ------------------------------------------------------------------------- */
+/* single-ray entry (tests): one step_map call in map units                    */
+
+typedef struct ORrayHit {
+    uint32_t hit, voxelIndex, sMap, sVox;
+    float pos[3];
+    float normal[3];
+} ORrayHit;
+
+void or_cast_rays(const ORscene* s, uint32_t n, const float* origins, const float* dirs, uint32_t ignoreFirst,
+                  ORrayHit* out, ORcounters* cnt)
+{
+    ORthread t;
+    thread_init(&t, 0);
+    for(uint32_t i = 0; i < n; i++)
+    {
+        v3 dir = vf(dirs + 3 * i), p = vf(origins + 3 * i), nrm = vs(0), ca;
+        real cm;
+        ORvoxel vox;
+        memset(&vox, 0, sizeof(vox));
+        t.lastVoxID = 255u; t.lastVoxRefract = RL(1.0);
+        int h = step_map(s, &t, &dir, vinv(dir), &p, (int)ignoreFirst, &nrm, &vox, &ca, &cm);
+        out[i].hit = (uint32_t)h;
+        out[i].voxelIndex = h ? t.hitIndex : 0xFFFFFFFFu;
+        out[i].sMap = t.sMap; out[i].sVox = t.sVox;
+        out[i].pos[0] = (float)p.x; out[i].pos[1] = (float)p.y; out[i].pos[2] = (float)p.z;
+        out[i].normal[0] = (float)nrm.x; out[i].normal[1] = (float)nrm.y; out[i].normal[2] = (float)nrm.z;
+    }
+    if(cnt) *cnt = t.cnt;
+}
+
+/* ------------------------------------------------------------------------- */
+/* draw cast (voxelDraw.comp:24-147). Raster compositing and the cubemap sky   */
+/* are not on this path (composeRasterized false, useCubemap false).           */
+
+typedef struct ORdrawParams {
+    float invView[16];        /* column-major m[col][row] (QuickMath layout) */
+    float invViewProj[16];    /* invCenteredView * invProjection, multiplied on the host */
+    uint32_t width, height;
+    uint32_t viewMode;
+} ORdrawParams;
+
+typedef struct ORpixel {
+    uint32_t voxelIndex, sMap, sVox, pad;
+    float face[3];
+} ORpixel;
+
+static v3 voxel_color(const ORscene* s, uint32_t viewMode, ORvoxel vox, v3 ca, real cm, v3 hitNormal)
+{
+    const ORmaterial* m = &s->mats[vox.material];
+    vox.spec = vmuls(vox.spec, (real)m->specular);
+    vox.diffuse = vmuls(vox.diffuse, RL(1.0) - (real)m->specular);
+    switch(viewMode)
+    {
+    case 0: {
+        v3 solid = m->emissive ? vmuls(vox.albedo, emission(m)) : vadd(vmul(vox.diffuse, vox.albedo), vox.spec);
+        return vadd(vmuls(solid, cm), ca);
+    }
+    case 1: return vox.albedo;
+    case 2: return m->emissive ? vox.albedo : vox.diffuse;
+    case 3: return m->emissive ? vox.albedo : vox.spec;
+    case 4: return vabs(vox.normal);
+    default: return vabs(hitNormal);
+    }
+}
+
+void or_draw(const ORscene* s, const ORdrawParams* dp, float* rgba, ORpixel* px, uint32_t* flagsOut, ORcounters* cnt)
+{
+    ORthread t;
+    thread_init(&t, 1);
+    const float* M = dp->invViewProj;
+    v3 ms = V((real)s->mapSize[0], (real)s->mapSize[1], (real)s->mapSize[2]);
+    for(uint32_t y = 0; y < dp->height; y++)
+    for(uint32_t x = 0; x < dp->width; x++)
+    {
+        t.lastVoxID = 255u; t.lastVoxRefract = RL(1.0);
+        size_t o = (size_t)y * dp->width + x;
+        real sx = (real)x / (real)dp->width * RL(2.0) - RL(1.0);
+        real sy = (real)y / (real)dp->height * RL(2.0) - RL(1.0);
+        v3 org = V((real)dp->invView[12], (real)dp->invView[13], (real)dp->invView[14]);
+        v3 p = org;
+        v3 q = V((real)M[0] * sx + (real)M[4] * sy + (real)M[8] * RL(0.0) + (real)M[12],
+                 (real)M[1] * sx + (real)M[5] * sy + (real)M[9] * RL(0.0) + (real)M[13],
+                 (real)M[2] * sx + (real)M[6] * sy + (real)M[10] * RL(0.0) + (real)M[14]);
+        v3 dir = vadds(vnorm(q), EPS);
+        v3 inv = vinv(dir);
+        v3 color;
+        real depth = RL(-1.0);
+        ORpixel info = {0xFFFFFFFFu, 0, 0, 0, {0, 0, 0}};
+        v3 tMin = vmul(vsub(vs(0), p), inv), tMax = vmul(vsub(ms, p), inv);
+        v3 t1 = vmin(tMin, tMax), t2 = vmax(tMin, tMax);
+        real tNear = gmax(gmax(t1.x, t1.y), t1.z), tFar = gmin(gmin(t2.x, t2.y), t2.z);
+        if(tNear > tFar || tFar < 0)
+            color = vpow(sky_color(s, dir), RL(2.2));
+        else
+        {
+            if(tNear > 0) p = vadd(p, vmuls(dir, tNear + EPS));
+            /* normal_AABB, voxelShared.comp:276-284 */
+            v3 c = vmuls(vadd(vs(0), ms), RL(0.5));
+            v3 pc = vsub(p, c);
+            v3 dd = vmuls(vsub(ms, vs(0)), RL(0.5));
+            real bias = RL(1.0) + EPS;
+            v3 tr = V(r_trunc(pc.x / dd.x * bias), r_trunc(pc.y / dd.y * bias), r_trunc(pc.z / dd.z * bias));
+            v3 nrm = vnorm(tr);
+            ORvoxel vox;
+            memset(&vox, 0, sizeof(vox));
+            v3 ca; real cm;
+            int hit = step_map(s, &t, &dir, inv, &p, 0, &nrm, &vox, &ca, &cm);
+            info.sMap = t.sMap; info.sVox = t.sVox;
+            if(hit)
+            {
+                v3 fp = vfloor(p);
+                if(flagsOut && fp.x >= 0 && fp.y >= 0 && fp.z >= 0 && fp.x < ms.x && fp.y < ms.y && fp.z < ms.z)
+                    flagsOut[map_index_u(s, (uint32_t)fp.x, (uint32_t)fp.y, (uint32_t)fp.z)] |= 4u;
+                v3 dv = vsub(p, org);
+                depth = r_sqrt(vdot(dv, dv));
+                color = voxel_color(s, dp->viewMode, vox, ca, cm, nrm);
+                info.voxelIndex = t.hitIndex;
+            }
+            else
+                color = vadd(vmuls(vpow(sky_color(s, dir), RL(2.2)), cm), ca);
+            info.face[0] = (float)nrm.x; info.face[1] = (float)nrm.y; info.face[2] = (float)nrm.z;
+        }
+        color = vpow(color, RL(0.4545));
+        rgba[4 * o + 0] = (float)color.x; rgba[4 * o + 1] = (float)color.y;
+        rgba[4 * o + 2] = (float)color.z; rgba[4 * o + 3] = (float)depth;
+        if(px) px[o] = info;
+    }
+    if(cnt) *cnt = t.cnt;
+}
+
+/* ------------------------------------------------------------------------- */
+/* hash RNG (voxelLighting.comp:29-57)                                         */
+
+static inline real rnd(real seed) { return fractr(r_sin(seed) * RL(43758.5453)) * RL(2.0) - RL(1.0); }
+static inline v3 rnd3(real seed) { return V(rnd(seed), rnd(seed * RL(2.0)), rnd(seed * RL(3.0))); }
+#define RUS_MAX_TRIES 256   /* the reference loop is unbounded; this cap is never reached in practice */
+static v3 rand_unit_sphere(real seed)
+{
+    v3 p = vs(0);
+    for(int k = 0; k < RUS_MAX_TRIES; k++)
+    {
+        p = rnd3(seed);
+        seed = seed + RL(1.0);
+        if(vdot(p, p) >= RL(1.0)) continue;
+        return p;
+    }
+    return p;
+}
+
+float or_rand(float seed) { return (float)rnd((real)seed); }
+void or_rand_unit_sphere(float seed, float* out)
+{
+    v3 p = rand_unit_sphere((real)seed);
+    out[0] = (float)p.x; out[1] = (float)p.y; out[2] = (float)p.z;
+}
+
+/* ------------------------------------------------------------------------- */
+/* lighting pass (voxelLighting.comp:65-288)                                   */
+
+typedef struct ORlightParams {
+    float time;
+    uint32_t numDiffuseSamples, maxDiffuseSamples, diffuseBounceLimit, specBounceLimit;
+    float sunDir[3];              /* normalised on the host (voxel.c:942) */
+    float ambient[3];
+    float shadowSoftness;
+    float camPos[3];
+    uint32_t numSpecRays;         /* 15 reproduces the reference's spherePoints */
+    uint32_t numDirectRays;       /* 0: one jittered shadow ray per sample (reference) */
+    const float* specPoints;      /* numSpecRays * 3 */
+    const float* directPoints;    /* numDirectRays * 3 */
+} ORlightParams;
+
+typedef struct LCtx {
+    const ORscene* s;
+    const ORlightParams* lp;
+    ORthread* t;
+    int firstSample;
+    v3 sunDir, sunStrength;
+    uint8_t* visProp;
+} LCtx;
+
+static void shadow_ray(LCtx* c, v3 p, real seed, v3* color)
+{
+    const ORlightParams* lp = c->lp;
+    ORvoxel hv; v3 ca, tn = vs(0); real cm;
+    memset(&hv, 0, sizeof(hv));
+    if(lp->numDirectRays == 0)
+    {
+        v3 dir = c->firstSample ? vadds(c->sunDir, EPS)
+                                : vadds(vnorm(vadd(vmuls(c->sunDir, (real)lp->shadowSoftness), rand_unit_sphere(seed))), EPS);
+        v3 pp = p;
+        if(!step_map(c->s, c->t, &dir, vinv(dir), &pp, 1, &tn, &hv, &ca, &cm))
+            *color = vadd(*color, vadd(vmuls(c->sunStrength, cm), ca));
+        return;
+    }
+    v3 acc = vs(0);
+    for(uint32_t k = 0; k < lp->numDirectRays; k++)
+    {
+        v3 dir = vadds(vnorm(vadd(vmuls(c->sunDir, (real)lp->shadowSoftness), vf(lp->directPoints + 3 * k))), EPS);
+        v3 pp = p;
+        if(!step_map(c->s, c->t, &dir, vinv(dir), &pp, 1, &tn, &hv, &ca, &cm))
+            acc = vadd(acc, vadd(vmuls(c->sunStrength, cm), ca));
+    }
+    *color = vadd(*color, vdivs(acc, (real)lp->numDirectRays));
+}
+
+static void specular_ray(LCtx* c, uint32_t thisMapIndex, v3 p, v3 dir, v3 albedo, uint32_t reflectType, v3* color)
+{
+    const ORscene* s = c->s;
+    ORvoxel hv; v3 tn = vs(0);
+    memset(&hv, 0, sizeof(hv));
+    v3 lastPos = p;
+    v3 mult = albedo;
+    for(uint32_t i = 0; i < c->lp->specBounceLimit; i++)
+    {
+        v3 ca; real cm;
+        if(step_map(s, c->t, &dir, vinv(dir), &p, 1, &tn, &hv, &ca, &cm))
+        {
+            /* ivec3(rayPos) truncates toward zero */
+            int32_t hx = (int32_t)p.x, hy = (int32_t)p.y, hz = (int32_t)p.z;
+            iv3 hp = {hx, hy, hz};
+            if(in_map(s, hp) && (s->map[(size_t)thisMapIndex * TILE_WORDS] & 4u) && c->visProp)
+                c->visProp[map_index_u(s, (uint32_t)hx, (uint32_t)hy, (uint32_t)hz)] = 1;
+            v3 dist = vabs(vsub(vfloor(vmuls(p, RL(8.0))), vfloor(vmuls(lastPos, RL(8.0)))));
+            if(vdot(dist, dist) <= RL(1.0)) return;
+            const ORmaterial* hm = &s->mats[hv.material];
+            hv.diffuse = vmuls(hv.diffuse, RL(1.0) - (real)hm->specular);
+            if(hm->emissive)
+            {
+                *color = vadd(*color, vmul(vmul(vadd(vmuls(vmuls(hv.albedo, emission(hm)), cm), ca), mult), albedo));
+                return;
+            }
+            v3 hitColor = vmul(hv.diffuse, hv.albedo);
+            *color = vadd(*color, vmul(vadd(vmuls(hitColor, cm), ca), mult));
+            if(hm->specular == 0.0f) return;
+            mult = vmul(mult, vmuls(vmuls(hv.albedo, cm), (real)hm->specular));
+            reflectType = hm->reflectType;
+            lastPos = p;
+            dir = vreflect(dir, hv.normal);
+        }
+        else if(vdot(dir, c->sunDir) > RL(0.99))
+        {
+            *color = vadd(*color, vadd(vmuls(c->sunStrength, cm), ca));
+            return;
+        }
+        else
+        {
+            v3 src = reflectType == 1u ? sky_color(s, dir) : c->sunStrength;
+            *color = vadd(*color, vmul(vadd(vmuls(src, cm), ca), mult));
+            return;
+        }
+    }
+}
+
+static void diffuse_ray(LCtx* c, v3 normal, v3 p, ORvoxel initial, real seed, v3* color)
+{
+    const ORscene* s = c->s;
+    const ORlightParams* lp = c->lp;
+    ORvoxel hv = initial;
+    v3 hitNormal = normal;
+    const ORmaterial* hm = NULL;
+    v3 newColor = vs(RL(1.0));
+    v3 lastPos = p;
+    v3 lastDir = vs(0);
+    for(uint32_t i = 0; i < lp->diffuseBounceLimit; i++)
+    {
+        v3 dir;
+        if(i > 0 && (rnd(seed + (real)lp->diffuseBounceLimit + (real)i) + RL(1.0)) * RL(0.5) < (real)hm->specular)
+            dir = vnorm(vadd(vmuls(vreflect(lastDir, hitNormal), (real)hm->shininess),
+                             rand_unit_sphere((real)lp->time + (real)i)));
+        else if(c->firstSample)
+            dir = vadds(vnorm(hitNormal), EPS);
+        else
+            dir = vadds(vnorm(vadd(hitNormal, rand_unit_sphere(seed + (real)i))), EPS);
+        v3 ca, tn = vs(0); real cm;
+        ORvoxel nv;
+        memset(&nv, 0, sizeof(nv));
+        int hit = step_map(s, c->t, &dir, vinv(dir), &p, 1, &tn, &nv, &ca, &cm);
+        if(hit)
+        {
+            hv = nv;
+            hitNormal = hv.normal;
+            hm = &s->mats[hv.material];
+            v3 dist = vabs(vsub(vfloor(vmuls(lastPos, RL(8.0))), vfloor(vmuls(p, RL(8.0)))));
+            if(vdot(dist, dist) < RL(1.0)) return;
+            if(hm->emissive)
+            {
+                *color = vadd(*color, vmul(newColor, vadd(vmuls(vmuls(hv.albedo, emission(hm)), cm), ca)));
+                return;
+            }
+            newColor = vmul(newColor, vadd(vmuls(hv.albedo, cm), ca));
+        }
+        else
+        {
+            real cosv = gmax(vdot(dir, c->sunDir), RL(0.0));
+            *color = vadd(*color, vadd(vmuls(vmul(vmuls(newColor, cosv), c->sunStrength), cm), ca));
+            return;
+        }
+        lastDir = dir;
+    }
+}
+
+static inline uint32_t rne_u(real x) { return (uint32_t)r_rint(x); }
+
+/* Runs one lighting dispatch over `requests`. Reads from the scene snapshot,
+ * writes lit voxel words into voxelsOut (a full copy of the pool), and applies
+ * the pass epilogue to mapOut / chunksOut (copies of map / chunks):
+ * visible bit cleared on lit chunks, then set on chunks reached by specular
+ * rays from visible chunks; sample count +1 on lit chunks.
+ * preq (optional, numRequests*32*6) receives the pre-quantisation diffuse and
+ * specular values, voxOf (optional, numRequests*32) the pool index per lane. */
+void or_lighting(const ORscene* s, const ORlightParams* lp, const uint32_t* requests, uint32_t numRequests,
+                 uint32_t* voxelsOut, uint32_t* mapOut, uint32_t* chunksOut, real* preq, uint32_t* voxOf,
+                 ORcounters* cnt)
+{
+    uint32_t tiles = s->mapSize[0] * s->mapSize[1] * s->mapSize[2];
+    uint8_t* prop = (uint8_t*)calloc(tiles ? tiles : 1, 1);
+    uint8_t* lit = (uint8_t*)calloc(tiles ? tiles : 1, 1);
+    uint32_t* flagsW = (uint32_t*)malloc(sizeof(uint32_t) * (tiles ? tiles : 1));
+    uint32_t* lastW = (uint32_t*)malloc(sizeof(uint32_t) * (tiles ? tiles : 1));
+    for(uint32_t i = 0; i < tiles; i++) { flagsW[i] = s->map[3 * i]; lastW[i] = s->map[3 * i + 1]; }
+    ORscene sc = *s;
+    sc.flagsOut = flagsW;
+    sc.lastUsedOut = lastW;
+    ORthread t;
+    thread_init(&t, 0);
+    LCtx c;
+    c.s = &sc; c.lp = lp; c.t = &t; c.visProp = prop;
+    c.sunDir = vf(lp->sunDir);
+    c.sunStrength = vf(s->sunStrength);
+    v3 amb = vf(lp->ambient);
+    for(uint32_t r = 0; r < numRequests; r++)
+    {
+        uint32_t mapIndex = requests[r] >> 4;
+        uint32_t block = requests[r] & 15u;
+        const uint32_t* chunk = s->chunks + (size_t)mapIndex * CHUNK_WORDS;
+        lit[mapIndex] = 1;
+        for(uint32_t lane = 0; lane < 32; lane++)
+        {
+            size_t slot = (size_t)r * 32 + lane;
+            if(voxOf) voxOf[slot] = 0xFFFFFFFFu;
+            uint32_t voxNum = lane + block * 32;
+            int32_t sel = or_select(chunk, voxNum);
+            if(sel < 0) continue;
+            t.lastVoxID = 255u; t.lastVoxRefract = RL(1.0);
+            iv3 cp = {sel % 8, (sel / 8) % 8, sel / 64};
+            iv3 mp = {(int32_t)chunk[0], (int32_t)chunk[1], (int32_t)chunk[2]};
+            uint32_t vi = s->map[(size_t)mapIndex * TILE_WORDS + 2] + (uint32_t)voxNum;
+            const uint32_t* cw = s->voxels + (size_t)vi * VOX_WORDS;
+            ORvoxel me = decompress(cw);
+            const ORmaterial* mat = &s->mats[me.material];
+            uint32_t ns = chunk[3];
+            real indirect = (real)(ns < lp->maxDiffuseSamples ? ns : lp->maxDiffuseSamples);
+            c.firstSample = indirect == 0;
+            v3 p = vadd(vadd(vmuls(vfromi(cp), RL(0.125)), vfromi(mp)), vs(RL(0.0625)));
+            p = vadd(p, vmul(vs(RL(0.0625) - EPS), me.normal));
+            v3 spec = vs(0), diff = vs(0);
+            v3 view = vsub(p, vf(lp->camPos));
+            if(mat->specular > 0.0f && vdot(view, me.normal) < 0 && mat->reflectType <= 1u)
+            {
+                v3 refl = vreflect(vnorm(view), me.normal);
+                for(uint32_t i = 0; i < lp->numSpecRays; i++)
+                {
+                    v3 sd = vadds(vnorm(vadd(vmuls(refl, (real)mat->shininess), vf(lp->specPoints + 3 * i))), EPS);
+                    specular_ray(&c, mapIndex, p, sd, me.albedo, mat->reflectType, &spec);
+                }
+                spec = vdivs(spec, (real)lp->numSpecRays);
+            }
+            if(mat->specular < 1.0f)
+            {
+                for(uint32_t i = 0; i < lp->numDiffuseSamples; i++)
+                {
+                    diff = vadd(diff, amb);
+                    diffuse_ray(&c, vadds(me.normal, EPS), p, me, (real)lp->time * (real)(i + 1), &diff);
+                    shadow_ray(&c, p, (real)lp->time * (real)(i + 1 + lp->numDiffuseSamples), &diff);
+                }
+                diff = vdivs(vadd(vmuls(me.diffuse, indirect), diff), indirect + (real)lp->numDiffuseSamples);
+            }
+            if(preq)
+            {
+                preq[slot * 6 + 0] = diff.x; preq[slot * 6 + 1] = diff.y; preq[slot * 6 + 2] = diff.z;
+                preq[slot * 6 + 3] = spec.x; preq[slot * 6 + 4] = spec.y; preq[slot * 6 + 5] = spec.z;
+            }
+            if(voxOf) voxOf[slot] = vi;
+            spec = vclamp(spec, RL(0.0), RL(1.0));
+            diff = vclamp(diff, RL(0.0), RL(1.0));
+            uint32_t wdx = rne_u(diff.x * RL(65535.0)), wdy = rne_u(diff.y * RL(65535.0)), wdz = rne_u(diff.z * RL(65535.0));
+            uint32_t ar = rne_u(me.albedo.x * RL(255.0)) & 255u, ag = rne_u(me.albedo.y * RL(255.0)) & 255u,
+                     ab = rne_u(me.albedo.z * RL(255.0)) & 255u;
+            uint32_t sx = rne_u(spec.x * RL(255.0)) & 255u, sy = rne_u(spec.y * RL(255.0)) & 255u,
+                     sz = rne_u(spec.z * RL(255.0)) & 255u;
+            uint32_t* ow = voxelsOut + (size_t)vi * VOX_WORDS;
+            ow[1] = (ar << 24) | (ag << 16) | (ab << 8) | sx;
+            ow[2] = (sy << 24) | (sz << 16) | (((wdx >> 8) & 255u) << 8) | (wdx & 255u);
+            ow[3] = (((wdy >> 8) & 255u) << 24) | ((wdy & 255u) << 16) | (((wdz >> 8) & 255u) << 8) | (wdz & 255u);
+        }
+    }
+    for(uint32_t i = 0; i < tiles; i++)
+    {
+        uint32_t f = flagsW[i];
+        if(lit[i]) f &= ~4u;
+        if(prop[i]) f |= 4u;
+        mapOut[3 * i] = f;
+        mapOut[3 * i + 1] = lastW[i];
+        mapOut[3 * i + 2] = s->map[3 * i + 2];
+        if(lit[i]) chunksOut[(size_t)i * CHUNK_WORDS + 3] = s->chunks[(size_t)i * CHUNK_WORDS + 3] + 1u;
+    }
+    if(cnt) *cnt = t.cnt;
+    free(prop); free(lit); free(flagsW); free(lastW);
+}
+
+/* ------------------------------------------------------------------------- */
+/* chunk conversion (voxel.c:1391-1461). Host C code in the reference: always  */
+/* FP32, whatever this build's real type is.                                   */
+
+/* cpuVoxels: DNchunk.voxels[8][8][8] of {normal, albedo}, flat index ((x*8)+y)*8+z.
+ * Writes chunkOut (24 words; pos and sample count left to the caller) and up to
+ * 512 voxels (4 words each); returns the number of stored voxels. */
+uint32_t or_chunk_to_gpu(const ORmaterial* mats, const uint32_t* cpuVoxels, uint32_t* chunkOut, uint32_t* voxOut)
+{
+    for(int i = 0; i < 16; i++) chunkOut[7 + i] = 0;
+    uint32_t n = 0;
+#define CV(x, y, z) (cpuVoxels + 2 * ((((x) * 8) + (y)) * 8 + (z)))
+#define FACE(x, y, z) (!((x) >= 0 && (x) < 8 && (y) >= 0 && (y) < 8 && (z) >= 0 && (z) < 8) || \
+                       (CV(x, y, z)[0] >> 24) == 255u || mats[CV(x, y, z)[0] >> 24].opacity < 1.0f)
+    for(int z = 0; z < 8; z++)
+    for(int y = 0; y < 8; y++)
+    for(int x = 0; x < 8; x++)
+    {
+        uint32_t index = (uint32_t)(x + 8 * (y + 8 * z));
+        if((index & 31u) == 0 && index != 0 && ((index >> 5) & 3u) == 0) chunkOut[4 + (index >> 7) - 1] = n;
+        const uint32_t* v = CV(x, y, z);
+        if((v[0] >> 24) == 255u) continue;
+        int vis = FACE(x + 1, y, z) || FACE(x - 1, y, z) || FACE(x, y + 1, z) || FACE(x, y - 1, z) ||
+                  FACE(x, y, z + 1) || FACE(x, y, z - 1);
+        if(!vis) continue;
+        chunkOut[7 + (index >> 5)] |= 1u << (index & 31u);
+        uint32_t a = v[1];
+        float lx = (float)((a >> 24) & 255u) * 0.00392156862f;
+        float ly = (float)((a >> 16) & 255u) * 0.00392156862f;
+        float lz = (float)((a >> 8) & 255u) * 0.00392156862f;
+        lx = powf(lx, 2.2f) * 255.0f; ly = powf(ly, 2.2f) * 255.0f; lz = powf(lz, 2.2f) * 255.0f;
+        uint8_t r = (uint8_t)lx, g = (uint8_t)ly, b = (uint8_t)lz;
+        uint32_t* o = voxOut + 4 * n;
+        o[0] = v[0];
+        o[1] = ((uint32_t)r << 24) | ((uint32_t)g << 16) | ((uint32_t)b << 8);
+        o[2] = 0; o[3] = 0;
+        n++;
+    }
+#undef FACE
+#undef CV
+    return n;
+}
+
+/* ------------------------------------------------------------------------- */
+/* visible-chunk request list (voxel.c:719-762, 1463-1489)                     */
+
+/* Tiles in z,y,x order with loaded flag 2 and the visible bit, that are in
+ * this frame's lighting split or were (re)uploaded this sync (fresh != 0),
+ * emit one word (mapIndex << 4) | block per 32 stored voxels. The stored
+ * voxel count comes from the chunk's occupancy mask. */
+uint32_t or_visible_requests(const uint32_t* map, const uint32_t* chunks, uint32_t tiles, const uint8_t* fresh,
+                             uint32_t split, uint32_t frameNum, uint32_t* out)
+{
+    uint32_t k = 0;
+    for(uint32_t i = 0; i < tiles; i++)
+    {
+        uint32_t f = map[3 * i];
+        if((f & 3u) != 2u || !(f & 4u)) continue;
+        if(i % split != frameNum && !(fresh && fresh[i])) continue;
+        uint32_t n = 0;
+        for(int w = 0; w < 16; w++) n += popc(chunks[(size_t)i * CHUNK_WORDS + 7 + w]);
+        for(uint32_t b = 0; b * 32 < n; b++) out[k++] = (i << 4) | b;
+    }
+    return k;
+}
+
+/* sizes of the exchanged structs, for the ctypes mirror */
+uint32_t or_sizeof(int which)
+{
+    switch(which)
+    {
+    case 0: return sizeof(ORmaterial);
+    case 1: return sizeof(ORscene);
+    case 2: return sizeof(ORcounters);
+    case 3: return sizeof(ORrayHit);
+    case 4: return sizeof(ORdrawParams);
+    case 5: return sizeof(ORpixel);
+    case 6: return sizeof(ORlightParams);
+    case 7: return sizeof(real);
+    default: return 0;
+    }
+}

@@ -1,0 +1,240 @@
+"""GPU parity: the HIP kernels through the C-ABI against the FP32 oracle.
+
+Each check exports the device state (in the reference's buffer layouts) right
+before a pass, runs the oracle on exactly that state, and compares:
+bit-exact for hit voxel index, face, step counts, request lists, stored
+lighting codes and pre-quantisation FP32 values; draw colours within 2e-6
+relative (the only non-bit-exact operation there is powf).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from doonengine_amd import scenes
+from doonengine_amd._hip import DeviceBuffer
+from doonengine_amd.volume import PIXELHIT_DTYPE, Volume, mat_to_np
+from helpers import build_state, camera, materials_array
+
+pytestmark = pytest.mark.gpu
+
+
+def load_scene(sc, min_chunks=64):
+    vol = Volume(sc.map_size, min_chunks=min_chunks)
+    sc.apply(vol)
+    vol.request_all()
+    vol.sync()
+    vol.synchronize()
+    return vol
+
+
+def oracle_scene(o, vol, sc, state=None):
+    m, c, v = state if state is not None else vol.export_state()
+    mats = vol.materials.copy()
+    s = o.scene(sc.map_size, m, c, v, mats, sun=tuple(getattr(vol.v.sunStrength, k) for k in "xyz"),
+                sky_bot=tuple(getattr(vol.v.skyGradientBot, k) for k in "xyz"),
+                sky_top=tuple(getattr(vol.v.skyGradientTop, k) for k in "xyz"))
+    return s, (m, c, v)
+
+
+def draw_gpu(vol, sc, w=None, h=None):
+    w = w or sc.width
+    h = h or sc.height
+    view, proj = vol.view_projection(h / w, 0.1, 100.0)
+    out = DeviceBuffer(w * h * 16)
+    hits = DeviceBuffer(w * h * PIXELHIT_DTYPE.itemsize)
+    vol.lib.DNX_set_draw_hit_output(vol.ptr, hits.ptr, w * h)
+    vol.draw(out, w, h, view, proj)
+    vol.synchronize()
+    vol.lib.DNX_set_draw_hit_output(vol.ptr, None, 0)
+    rgba = out.download(np.float32, (h, w, 4))
+    px = hits.download(PIXELHIT_DTYPE, (h, w))
+    return rgba, px, view, proj
+
+
+def sun_dir_normalized(vol):
+    d = np.array([vol.v.sunDir.x, vol.v.sunDir.y, vol.v.sunDir.z], np.float32)
+    n = np.float32(np.sqrt(np.float32(d[0] * d[0] + d[1] * d[1]) + np.float32(d[2] * d[2])))
+    return d * (np.float32(1.0) / n) if n != 0 else np.zeros(3, np.float32)
+
+
+@pytest.fixture(scope="module")
+def small_scene():
+    return scenes.config_small()
+
+
+def test_upload_matches_oracle_conversion(gpu, small_scene):
+    sc = small_scene
+    vol = load_scene(sc)
+    m, c, v = vol.export_state()
+    rm, rc, rv, _ = build_state(sc, materials_array(sc))
+    loaded = (m[:, 0] & 3) == 2
+    assert np.array_equal(loaded, (rm[:, 0] & 3) == 2)
+    assert np.array_equal(c[loaded], rc[loaded])          # pos, samples 0, partialCounts, bitMask
+    for t in np.nonzero(loaded)[0]:
+        n = int(sum(bin(int(w)).count("1") for w in c[t, 7:23]))
+        assert np.array_equal(v[m[t, 2]:m[t, 2] + n], rv[rm[t, 2]:rm[t, 2] + n])
+    vol.close()
+
+
+def test_draw_parity(gpu, small_scene):
+    sc = small_scene
+    o = oracle.load("f32")
+    vol = load_scene(sc)
+    s, (m, c, v) = oracle_scene(o, vol, sc)
+    rgba, px, view, proj = draw_gpu(vol, sc)
+    from doonengine_amd.volume import draw_matrices
+    iv, ivp = draw_matrices(view, proj)
+    flags = m[:, 0].copy()
+    orgba, opx, ocnt = o.draw(s, mat_to_np(iv), mat_to_np(ivp), sc.width, sc.height, flags_out=flags)
+    assert (px["voxelIndex"] != 0xFFFFFFFF).mean() > 0.2
+    assert np.array_equal(px["voxelIndex"], opx["voxelIndex"])
+    assert np.array_equal(px["mapSteps"], opx["mapSteps"])
+    assert np.array_equal(px["voxelSteps"], opx["voxelSteps"])
+    assert np.array_equal(px["face"].view(np.uint32) & 0x7FFFFFFF, opx["face"].view(np.uint32) & 0x7FFFFFFF)
+    assert np.array_equal(rgba[..., 3], orgba[..., 3])    # depth: sqrt and arithmetic only
+    np.testing.assert_allclose(rgba[..., :3], orgba[..., :3], rtol=2e-6, atol=1e-7)
+    m2, _, _ = vol.export_state()
+    assert np.array_equal(m2[:, 0], flags)                # visible bits
+    cnt = vol.counters(0, reset=True)
+    assert cnt["rays"] == ocnt["calls"] and cnt["mapSteps"] == ocnt["sMap"] and cnt["voxelSteps"] == ocnt["sVox"]
+    vol.close()
+
+
+def test_rays_parity(gpu, small_scene):
+    sc = small_scene
+    o = oracle.load("f32")
+    vol = load_scene(sc)
+    s, _ = oracle_scene(o, vol, sc)
+    rng = np.random.default_rng(5)
+    n = 20000
+    X, Y, Z = sc.map_size
+    org = rng.uniform(-0.5, 1.0, (n, 3)) * np.array([X, Y, Z]) + 0.25
+    d = rng.normal(size=(n, 3))
+    # axis-parallel and diagonal directions (ties in the DDA)
+    d[: n // 10] = np.eye(3)[rng.integers(0, 3, n // 10)] * rng.choice([-1, 1], (n // 10, 1))
+    d[n // 10: n // 5] = rng.choice([-1.0, 1.0], (n // 10, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d = (d + 1e-4).astype(np.float32)
+    org = org.astype(np.float32)
+    # integer-aligned origins (start exactly on cell boundaries)
+    org[-1000:] = np.floor(org[-1000:])
+    for ignore in (False, True):
+        g = vol.cast_rays(org, d, ignore_first=ignore)
+        r, _ = o.cast_rays(s, org, d, ignore_first=ignore)
+        for f in ("hit", "voxelIndex", "mapSteps", "voxelSteps"):
+            assert np.array_equal(g[f], r[f]), f
+        assert np.array_equal(g["pos"].view(np.uint32), r["pos"].view(np.uint32))
+        assert np.array_equal(g["normal"].view(np.uint32) & 0x7FFFFFFF, r["normal"].view(np.uint32) & 0x7FFFFFFF)
+        assert g["hit"].mean() > 0.1
+    vol.close()
+
+
+def run_frame_and_compare(vol, sc, o, frame, time, w, h, num_diffuse=1, max_diffuse=1000, split=1):
+    """draw -> sync(READ) -> lighting on the GPU, each pass checked against the oracle."""
+    s, (m0, c0, v0) = oracle_scene(o, vol, sc)
+    rgba, px, view, proj = draw_gpu(vol, sc, w, h)
+    from doonengine_amd.volume import draw_matrices
+    iv, ivp = draw_matrices(view, proj)
+    flags = m0[:, 0].copy()
+    orgba, opx, _ = o.draw(s, mat_to_np(iv), mat_to_np(ivp), w, h, flags_out=flags)
+    assert np.array_equal(px["voxelIndex"], opx["voxelIndex"])
+    # visible-chunk pass
+    m1, c1, v1 = vol.export_state()
+    assert np.array_equal(m1[:, 0], flags)
+    vol.sync(op=2, split=split)
+    req = vol.fetch_requests()
+    ref_req = o.visible_requests(m1, c1, split=split, frame=int(vol.v.frameNum))
+    assert np.array_equal(req, ref_req)
+    m2, c2, v2 = vol.export_state()
+    assert np.array_equal(m2[:, 1], m1[:, 1] + 1)          # lastUsed++ on every tile
+    # lighting pass
+    R = max(len(req), 1)
+    dbg = DeviceBuffer(R * 32 * 6 * 4)
+    dbi = DeviceBuffer(R * 32 * 4)
+    vol.lib.DNX_set_lighting_debug_output(vol.ptr, dbg.ptr, dbi.ptr, R)
+    s2, _ = oracle_scene(o, vol, sc, (m2, c2, v2))
+    vol.update_lighting(num_diffuse, max_diffuse, time)
+    vol.synchronize()
+    vol.lib.DNX_set_lighting_debug_output(vol.ptr, None, None, 0)
+    g_val = dbg.download(np.float32, (R * 32, 6))[: len(req) * 32]
+    g_idx = dbi.download(np.uint32, (R * 32,))[: len(req) * 32]
+    spec_pts = vol.ray_points(0)
+    direct = vol.ray_points(1) if vol.v.numDirectRays else None
+    vo, mo, co, preq, voxof, ocnt = o.lighting(
+        s2, req, time=float(vol.v.lastTime), num_diffuse=num_diffuse, max_diffuse=max_diffuse,
+        diffuse_bounces=int(vol.v.diffuseBounceLimit), spec_bounces=int(vol.v.specBounceLimit),
+        sun_dir=sun_dir_normalized(vol),
+        ambient=tuple(getattr(vol.v.ambientLightStrength, k) for k in "xyz"), softness=float(vol.v.shadowSoftness),
+        cam_pos=tuple(getattr(vol.v.camPos, k) for k in "xyz"), spec_points=spec_pts, direct_points=direct)
+    m3, c3, v3 = vol.export_state()
+    assert np.array_equal(g_idx, voxof)
+    valid = voxof != 0xFFFFFFFF
+    assert np.array_equal(g_val[valid].view(np.uint32), preq[valid].astype(np.float32).view(np.uint32))
+    assert np.array_equal(v3, vo)
+    assert np.array_equal(m3[:, 0], mo[:, 0])
+    assert np.array_equal(c3[:, 3], co[:, 3])
+    loaded = (m3[:, 0] & 3) == 2
+    assert np.array_equal(m3[loaded, 1], mo[loaded, 1])    # lastUsed of loaded chunks
+    cnt = vol.counters(1, reset=True)
+    assert cnt["rays"] == ocnt["calls"] and cnt["mapSteps"] == ocnt["sMap"] and cnt["voxelSteps"] == ocnt["sVox"]
+    assert cnt["litVoxels"] == int(valid.sum())
+    return len(req), int(valid.sum())
+
+
+def test_lighting_parity_frames(gpu, small_scene):
+    sc = small_scene
+    o = oracle.load("f32")
+    vol = load_scene(sc)
+    total = 0
+    for f in range(4):
+        nreq, nlit = run_frame_and_compare(vol, sc, o, f, 123.456 + f / 60.0, sc.width, sc.height)
+        total += nlit
+    assert total > 1000
+    vol.close()
+
+
+def test_lighting_parity_specular_and_direct(gpu):
+    sc = scenes.config_small(seed=12)
+    sc.materials[1].specular = 0.7
+    sc.materials[1].shininess = 3
+    sc.materials[2].specular = 1.0
+    sc.materials[2].shininess = 100
+    sc.materials[2].reflect_type = 1
+    o = oracle.load("f32")
+    vol = load_scene(sc)
+    vol.v.numDirectRays = 8
+    vol.v.numSpecularRays = 15
+    for f in range(3):
+        run_frame_and_compare(vol, sc, o, f, 200.0 + f / 60.0, sc.width, sc.height)
+    vol.v.numSpecularRays = 16
+    run_frame_and_compare(vol, sc, o, 3, 201.0, sc.width, sc.height)
+    vol.close()
+
+
+def test_lighting_split_and_samples(gpu, small_scene):
+    sc = small_scene
+    o = oracle.load("f32")
+    vol = load_scene(sc)
+    for f in range(4):
+        run_frame_and_compare(vol, sc, o, f, 300.0 + f, sc.width, sc.height, num_diffuse=2, max_diffuse=2, split=3)
+    vol.close()
+
+
+def test_transparent_draw_parity(gpu):
+    sc = scenes.config_small(seed=13)
+    sc.materials[0].opacity = 0.4
+    o = oracle.load("f32")
+    vol = load_scene(sc)
+    vol.materials[0]["refractIndex"] = 1.3
+    s, (m, c, v) = oracle_scene(o, vol, sc)
+    rgba, px, view, proj = draw_gpu(vol, sc)
+    from doonengine_amd.volume import draw_matrices
+    iv, ivp = draw_matrices(view, proj)
+    orgba, opx, _ = o.draw(s, mat_to_np(iv), mat_to_np(ivp), sc.width, sc.height)
+    assert np.array_equal(px["voxelIndex"], opx["voxelIndex"])
+    assert np.array_equal(px["mapSteps"], opx["mapSteps"])
+    assert np.array_equal(px["voxelSteps"], opx["voxelSteps"])
+    np.testing.assert_allclose(rgba[..., :3], orgba[..., :3], rtol=2e-6, atol=1e-7)
+    vol.close()
