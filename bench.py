@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X voxel pipeline on the full-frame workload.
+
+One step = one frame of the engine loop (src/main.c:503-505): DN_draw of a
+1920x1080 primary-ray cast, DN_sync_gpu (visible-chunk pass), DN_update_lighting
+over the visible chunks. Workload: 64x32x64 chunks (512x256x512 voxels), fBm
+terrain plus 2000 seeded boxes (~30% occupancy, ~64% empty chunks), 32 fixed
+direct rays per diffuse sample, <= 4 diffuse bounces, FP32. The lighting load is
+the configured "up to 32768 visible chunks": besides the chunks the draw marks
+visible, DNX_mark_visible marks the first 32768 loaded chunks visible each frame.
+
+Prints ONE JSON line. `value` is DDA rays/s (step_map calls of draw + lighting
+per second); voxel-lighting updates/s, per-kernel times, the lighting kernel's
+roofline (algorithmic bytes / HIP-event duration vs 8 TB/s) and a CPU baseline
+(the oracle on a bounded sample, on this host) ride along.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+VISIBLE_CAP = 32768
+
+
+def algorithmic_bytes(c: dict) -> float:
+    """Bytes per the traversal model: a chunk-grid trip reads a 12-byte tile,
+    a voxel trip a 4-byte mask word, a hit 16 bytes of voxel plus 20 of rank
+    data; each lit voxel reads 16, writes 12, plus its share of the chunk
+    header (96/32) and request word (4/32)."""
+    return (12.0 * c["mapSteps"] + 4.0 * c["voxelSteps"] + 36.0 * c["hits"]
+            + c.get("litVoxels", 0) * (16.0 + 12.0 + 96.0 / 32.0 + 4.0 / 32.0))
+
+
+def cpu_baseline(vol, sc, budget_s: float) -> dict:
+    """Times the FP32 oracle (single thread) on a bounded prefix of the frame's
+    lighting requests and a band of draw rows, on the exported device state."""
+    import numpy as np
+
+    import oracle
+    from doonengine_amd.volume import draw_matrices, mat_to_np
+
+    o = oracle.load("f32")
+    out = __import__("doonengine_amd._hip", fromlist=["DeviceBuffer"]).DeviceBuffer(sc.width * sc.height * 16)
+    view, proj = vol.view_projection(sc.height / sc.width)
+    vol.draw(out, sc.width, sc.height, view, proj)
+    vol.mark_visible(VISIBLE_CAP)
+    vol.sync()
+    req = vol.fetch_requests()
+    m2, c2, v2 = vol.export_state()
+    s = o.scene(sc.map_size, m2, c2, v2, vol.materials.copy(),
+                sun=tuple(getattr(vol.v.sunStrength, k) for k in "xyz"))
+    sd = np.array([vol.v.sunDir.x, vol.v.sunDir.y, vol.v.sunDir.z], np.float32)
+    sd = sd * (np.float32(1.0) / np.sqrt(np.float32(sd[0] * sd[0] + sd[1] * sd[1]) + np.float32(sd[2] * sd[2])))
+    kw = dict(time=float(vol.v.lastTime), diffuse_bounces=int(vol.v.diffuseBounceLimit), sun_dir=sd,
+              spec_points=vol.ray_points(0), direct_points=vol.ray_points(1) if vol.v.numDirectRays else None,
+              cam_pos=tuple(getattr(vol.v.camPos, k) for k in "xyz"))
+    # calibrate on a small prefix, then scale to the budget
+    n = min(len(req), 64)
+    t0 = time.perf_counter()
+    o.lighting(s, req[:n], **kw)
+    dt = max(time.perf_counter() - t0, 1e-3)
+    n = int(min(len(req), max(n, n * 0.6 * budget_s / dt)))
+    t0 = time.perf_counter()
+    _, _, _, _, voxof, lc = o.lighting(s, req[:n], **kw)
+    lt = time.perf_counter() - t0
+    rows = max(8, min(sc.height, int(sc.height * 0.3 * budget_s / max(lt, 1e-3) * n / max(len(req), 1))))
+    view, proj = vol.view_projection(sc.height / sc.width)
+    iv, ivp = draw_matrices(view, proj)
+    t0 = time.perf_counter()
+    _, _, dc = o.draw(s, mat_to_np(iv), mat_to_np(ivp), sc.width, rows)
+    dtime = time.perf_counter() - t0
+    rays = lc["calls"] + dc["calls"]
+    lit = int((voxof != 0xFFFFFFFF).sum())
+    return {"value": rays / (lt + dtime), "unit": "rays/s", "cores": 1, "kind": "port",
+            "sample": f"FP32 oracle, 1 thread: lighting of the first {n} of {len(req)} requests "
+                      f"({lit} voxels, {lc['calls']} rays, {lt:.1f} s) + draw of {rows} rows x {sc.width} "
+                      f"({dc['calls']} rays, {dtime:.1f} s)",
+            "lighting_updates_per_s": lit / lt}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
+              "(multi-GPU sharding is design-only)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    from doonengine_amd import scenes
+    from doonengine_amd._hip import DeviceBuffer
+    from doonengine_amd.volume import KIND_DRAW, KIND_LIGHTING, Volume
+
+    t_setup = time.perf_counter()
+    sc = scenes.config2()
+    vol = Volume(sc.map_size, min_chunks=65536)
+    sc.apply(vol)
+    vol.request_all()
+    vol.sync()
+    vol.synchronize()
+    print(f"[bench] scene ready in {time.perf_counter() - t_setup:.1f} s: occupancy {sc.occupancy:.3f}, "
+          f"pool {int(vol.v.voxelCap)} voxels", file=sys.stderr, flush=True)
+    W, H = sc.width, sc.height
+    out = DeviceBuffer(W * H * 16)
+    view, proj = vol.view_projection(H / W)
+
+    frame = [0]
+
+    def step():
+        vol.draw(out, W, H, view, proj)
+        vol.mark_visible(VISIBLE_CAP)
+        vol.sync()
+        vol.update_lighting(1, 1000, 123.456 + frame[0] / 60.0)
+        frame[0] += 1
+
+    for _ in range(args.warmup):
+        step()
+    vol.synchronize()
+    print(f"[bench] warmup done ({args.warmup} frames)", file=sys.stderr, flush=True)
+    vol.counters(KIND_DRAW, reset=True)
+    vol.counters(KIND_LIGHTING, reset=True)
+    vol.enable_timing(True)
+    vol.timing(reset=True)
+    vol.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    vol.synchronize()
+    elapsed = time.perf_counter() - t0
+    vol.enable_timing(False)
+    tm = vol.timing(reset=True)
+    dc = vol.counters(KIND_DRAW)
+    lc = vol.counters(KIND_LIGHTING)
+    K = args.steps
+    rays = dc["rays"] + lc["rays"]
+    lit_per_frame = lc["litVoxels"] / K
+    light_ms = tm["lightingMs"] / max(tm["lightingLaunches"], 1)
+    light_bytes = algorithmic_bytes(lc) / K
+    achieved = light_bytes / (light_ms * 1e-3) / 1e9
+    print(f"[bench] {K} frames in {elapsed:.3f} s; lighting {light_ms:.3f} ms/launch, draw "
+          f"{tm['drawMs'] / max(tm['drawLaunches'], 1):.3f} ms/launch, {lit_per_frame:.0f} voxels lit/frame",
+          file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
+        cpu = cpu_baseline(vol, sc, args.cpu_seconds)
+
+    result = {
+        "metric": "DDA rays/s & voxel-lighting updates/s; cache vs plain frames-to-1%-RMSE",
+        "value": rays / elapsed,
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "full frame: 64x32x64 chunks (512x256x512 voxels), fBm terrain + 2000 boxes, "
+                        "1920x1080 primary DDA draw, lighting over <= 32768 visible chunks, 32 direct/specular "
+                        "rays, 1 diffuse sample (<= 4 bounces), FP32",
+            "map_chunks": list(sc.map_size),
+            "image": [W, H],
+            "visible_chunks_cap": VISIBLE_CAP,
+            "parallelism": "single GPU",
+        },
+        "voxel_lighting_updates_per_s": lc["litVoxels"] / elapsed,
+        "draw_rays_per_s": dc["rays"] / elapsed,
+        "lighting_rays_per_s": lc["rays"] / elapsed,
+        "per_frame": {"draw_rays": dc["rays"] / K, "lighting_rays": lc["rays"] / K, "voxels_lit": lit_per_frame},
+        "kernel_ms": {"lighting": light_ms, "draw": tm["drawMs"] / max(tm["drawLaunches"], 1),
+                      "visible": tm["visibleMs"] / max(tm["visibleLaunches"], 1),
+                      "epilogue": tm["epilogueMs"] / max(tm["epilogueLaunches"], 1)},
+        "roofline": {"kernel": "k_lighting", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": light_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(result), flush=True)
+    vol.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -142,3 +142,4 @@ int dn_launch_upload(uint32_t* flags, uint32_t* lastUsed, DNchunkRec* chunks, ui
 int dn_launch_rays(const DNrayParams& p, hipStream_t stream);
 int dn_launch_collect_requested(const uint32_t* flags, uint32_t tiles, uint32_t* list, uint32_t* count,
                                 hipStream_t stream);
+int dn_launch_mark_visible(uint32_t* flags, uint32_t tiles, uint32_t lastTile, hipStream_t stream);

@@ -1516,6 +1516,18 @@ extern "C" void DNX_set_voxels_dense(DNvolume* vol, DNivec3 origin, DNuvec3 size
 
 extern "C" void DNX_request_all_chunks(DNvolume* vol) { vol->gpu->requestAll = true; }
 
+extern "C" uint32_t DNX_mark_visible(DNvolume* vol, uint32_t maxChunks)
+{
+    DNgpuVolume* g = vol->gpu;
+    uint32_t n = 0, last = 0;
+    for(uint32_t t = 0; t < g->tiles && n < maxChunks; t++)
+        if(g->state[t] == 2) { n++; last = t; }
+    if(n == 0) return 0;
+    int e = dn_launch_mark_visible(g->dFlags, g->tiles, last, g->stream);
+    if(e) dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR, "mark visible launch failed (%d)", e);
+    return n;
+}
+
 extern "C" void* DNX_get_stream(DNvolume* vol) { return (void*)vol->gpu->stream; }
 
 extern "C" bool DNX_synchronize(DNvolume* vol)

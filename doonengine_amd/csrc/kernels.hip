@@ -536,6 +536,17 @@ __global__ __launch_bounds__(256) void k_collect_requested(const uint32_t* flags
     if(t < tiles && (flags[t] & 3u) == 3u) list[atomicAdd(count, 1u)] = t;
 }
 
+// Sets the visible bit of every loaded tile with index <= lastTile.
+__global__ __launch_bounds__(256) void k_mark_visible(uint32_t* flags, uint32_t tiles, uint32_t lastTile)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if(t < tiles && t <= lastTile)
+    {
+        uint32_t f = flags[t];
+        if((f & 3u) == 2u && !(f & 4u)) flags[t] = f | 4u;
+    }
+}
+
 int g_lightingBlocks = 0;
 
 }  // namespace
@@ -613,5 +624,12 @@ int dn_launch_collect_requested(const uint32_t* flags, uint32_t tiles, uint32_t*
 {
     if(tiles == 0) return 0;
     hipLaunchKernelGGL(k_collect_requested, dim3((tiles + 255) / 256), dim3(256), 0, stream, flags, tiles, list, count);
+    return (int)hipGetLastError();
+}
+
+int dn_launch_mark_visible(uint32_t* flags, uint32_t tiles, uint32_t lastTile, hipStream_t stream)
+{
+    if(tiles == 0) return 0;
+    hipLaunchKernelGGL(k_mark_visible, dim3((tiles + 255) / 256), dim3(256), 0, stream, flags, tiles, lastTile);
     return (int)hipGetLastError();
 }

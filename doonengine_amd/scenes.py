@@ -220,8 +220,8 @@ def config1(reference_like: bool = False) -> Scene:
 def config2() -> Scene:
     """64x32x64 chunks (512x256x512 voxels), fBm terrain + 2000 boxes (sizes 2-40),
     ~30% occupancy; 1920x1080 draw; 32 direct/specular rays, <= 4 diffuse bounces."""
-    nw, aw, mats = terrain((512, 256, 512), 0.30, seed=1, amplitude=64.0, n_boxes=2000, box_seed=3,
+    nw, aw, mats = terrain((512, 256, 512), 0.22, seed=1, amplitude=64.0, n_boxes=2000, box_seed=3,
                            box_size=(2, 40), n_materials=4, emission=4.0)
     params = {"numSpecularRays": 32, "numDirectRays": 32, "diffuseBounceLimit": 4}
-    return Scene("config2", (64, 32, 64), nw, aw, mats, cam_pos=(-2.0, 14.0, -2.0), cam_orient=(25.0, 45.0, 0.0),
+    return Scene("config2", (64, 32, 64), nw, aw, mats, cam_pos=(4.0, 12.0, 4.0), cam_orient=(30.0, 45.0, 0.0),
                  width=1920, height=1080, params=params)

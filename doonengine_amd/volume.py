@@ -115,6 +115,9 @@ class Volume:
     def request_all(self) -> None:
         self.lib.DNX_request_all_chunks(self.ptr)
 
+    def mark_visible(self, max_chunks: int) -> int:
+        return int(self.lib.DNX_mark_visible(self.ptr, int(max_chunks)))
+
     # -- pipeline ---------------------------------------------------------
     def sync(self, op: int = DN_READ_WRITE, split: int = 1) -> None:
         self.lib.DN_sync_gpu(self.ptr, op, split)
