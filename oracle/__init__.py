@@ -92,6 +92,12 @@ class Oracle:
         c = np.ascontiguousarray(chunk, np.uint32)
         return self.lib.or_select(c.ctypes.data_as(c_void_p), c_uint32(n))
 
+    def decompress(self, words) -> np.ndarray:
+        w = np.ascontiguousarray(words, np.uint32)
+        out = np.zeros(13, self.real)
+        self.lib.or_decompress(w.ctypes.data_as(c_void_p), out.ctypes.data_as(c_void_p))
+        return out
+
     def rand(self, seed: float) -> float:
         return self.lib.or_rand(float(seed))
 

@@ -913,3 +913,14 @@ uint32_t or_sizeof(int which)
     default: return 0;
     }
 }
+
+/* decompress_voxel for tests: out = normal(3), albedo(3), spec(3), diffuse(3), material */
+void or_decompress(const uint32_t* w, real* out)
+{
+    ORvoxel v = decompress(w);
+    out[0] = v.normal.x; out[1] = v.normal.y; out[2] = v.normal.z;
+    out[3] = v.albedo.x; out[4] = v.albedo.y; out[5] = v.albedo.z;
+    out[6] = v.spec.x; out[7] = v.spec.y; out[8] = v.spec.z;
+    out[9] = v.diffuse.x; out[10] = v.diffuse.y; out[11] = v.diffuse.z;
+    out[12] = (real)v.material;
+}
