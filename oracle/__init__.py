@@ -98,6 +98,18 @@ class Oracle:
         self.lib.or_decompress(w.ctypes.data_as(c_void_p), out.ctypes.data_as(c_void_p))
         return out
 
+    def set_rand_override(self, seeds=None, values=None) -> None:
+        """Use `values` as the hash of exactly these `seeds` (tests only); None clears."""
+        if seeds is None:
+            self._ov = None
+            self.lib.or_set_rand_override(None, None, c_uint32(0))
+            return
+        order = np.argsort(np.asarray(seeds, np.float32), kind="stable")
+        s = np.ascontiguousarray(np.asarray(seeds, np.float32)[order])
+        v = np.ascontiguousarray(np.asarray(values, np.float32)[order])
+        self._ov = (s, v)
+        self.lib.or_set_rand_override(s.ctypes.data_as(c_void_p), v.ctypes.data_as(c_void_p), c_uint32(len(s)))
+
     def rand(self, seed: float) -> float:
         return self.lib.or_rand(float(seed))
 

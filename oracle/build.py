@@ -16,12 +16,24 @@ FLAGS = ["-O2", "-fPIC", "-shared", "-std=c11", "-ffp-contract=off", "-fno-fast-
          "-Wall", "-Wextra", "-Wno-unused-function"]
 
 
+REFERENCE_SHADERS = Path("/root/reference/assets/shaders")
+
+
 def build() -> list[Path]:
     outs = []
     for name, extra in (("liboracle_f32.so", []), ("liboracle_f64.so", ["-DOR_DOUBLE"])):
         out = HERE / name
         cmd = ["gcc"] + FLAGS + extra + [str(SRC), "-o", str(out), "-lm"]
         subprocess.run(cmd, check=True)
+        outs.append(out)
+    # The llvmpipe harness that runs the reference's own shaders (read at run
+    # time from the reference tree) is only useful where that tree exists.
+    if REFERENCE_SHADERS.exists() and Path("/usr/include/GL/internal/dri_interface.h").exists():
+        ref = HERE / "_ref"
+        ref.mkdir(exist_ok=True)
+        out = ref / "libref_llvmpipe.so"
+        subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-Wall", "-Wno-missing-field-initializers",
+                        str(HERE / "ref_llvmpipe.c"), "-o", str(out), "-ldl"], check=True)
         outs.append(out)
     return outs
 

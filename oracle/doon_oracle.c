@@ -553,7 +553,32 @@ void or_draw(const ORscene* s, const ORdrawParams* dp, float* rgba, ORpixel* px,
 /* ------------------------------------------------------------------------- */
 /* hash RNG (voxelLighting.comp:29-57)                                         */
 
-static inline real rnd(real seed) { return fractr(r_sin(seed) * RL(43758.5453)) * RL(2.0) - RL(1.0); }
+/* Optional override of the hash for given seeds (tests only): lets a check feed
+ * the values another implementation's sin produced for exactly these seeds. */
+static const float* g_ovSeeds = NULL;
+static const float* g_ovValues = NULL;
+static uint32_t g_ovCount = 0;
+
+void or_set_rand_override(const float* seeds, const float* values, uint32_t n)
+{
+    g_ovSeeds = seeds; g_ovValues = values; g_ovCount = seeds && values ? n : 0;
+}
+
+static inline real rnd(real seed)
+{
+    if(g_ovCount)
+    {
+        float fs = (float)seed;
+        uint32_t lo = 0, hi = g_ovCount;   /* seeds sorted ascending */
+        while(lo < hi)
+        {
+            uint32_t mid = (lo + hi) / 2;
+            if(g_ovSeeds[mid] < fs) lo = mid + 1; else hi = mid;
+        }
+        if(lo < g_ovCount && g_ovSeeds[lo] == fs && (real)fs == seed) return (real)g_ovValues[lo];
+    }
+    return fractr(r_sin(seed) * RL(43758.5453)) * RL(2.0) - RL(1.0);
+}
 static inline v3 rnd3(real seed) { return V(rnd(seed), rnd(seed * RL(2.0)), rnd(seed * RL(3.0))); }
 #define RUS_MAX_TRIES 256   /* the reference loop is unbounded; this cap is never reached in practice */
 static v3 rand_unit_sphere(real seed)
