@@ -7,7 +7,9 @@ over the visible chunks. Workload: 64x32x64 chunks (512x256x512 voxels), fBm
 terrain plus 2000 seeded boxes (~30% occupancy, ~64% empty chunks), 32 fixed
 direct rays per diffuse sample, <= 4 diffuse bounces, FP32. The lighting load is
 the configured "up to 32768 visible chunks": besides the chunks the draw marks
-visible, DNX_mark_visible marks the first 32768 loaded chunks visible each frame.
+visible, DNX_mark_visible marks every exposed chunk (one holding a voxel with an
+empty face neighbour; 16201 in this scene) visible each frame, a full relight of
+everything that can be seen from somewhere.
 
 Prints ONE JSON line. `value` is DDA rays/s (step_map calls of draw + lighting
 per second); voxel-lighting updates/s, per-kernel times, the lighting kernel's
@@ -93,6 +95,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="after the timed frames, copy BYTES device-to-device once (a plain copy kernel of known "
+                         "size, to scale profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -155,6 +160,14 @@ def main() -> int:
           f"{tm['drawMs'] / max(tm['drawLaunches'], 1):.3f} ms/launch, {lit_per_frame:.0f} voxels lit/frame",
           file=sys.stderr, flush=True)
 
+    if args.copy_calibration > 0:
+        from doonengine_amd import _hip
+        a, b = DeviceBuffer(args.copy_calibration), DeviceBuffer(args.copy_calibration)
+        a.zero()
+        _hip.check(_hip.hip().hipMemcpy(b.ptr, a.ptr, args.copy_calibration, 3), "calibration copy")
+        _hip.synchronize()
+        print(f"[bench] calibration copy of {args.copy_calibration} bytes done", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
         cpu = cpu_baseline(vol, sc, args.cpu_seconds)
@@ -184,7 +197,10 @@ def main() -> int:
         "voxel_lighting_updates_per_s": lc["litVoxels"] / elapsed,
         "draw_rays_per_s": dc["rays"] / elapsed,
         "lighting_rays_per_s": lc["rays"] / elapsed,
-        "per_frame": {"draw_rays": dc["rays"] / K, "lighting_rays": lc["rays"] / K, "voxels_lit": lit_per_frame},
+        "per_frame": {"draw_rays": dc["rays"] / K, "lighting_rays": lc["rays"] / K, "voxels_lit": lit_per_frame,
+                      "lighting_map_steps": lc["mapSteps"] / K, "lighting_voxel_steps": lc["voxelSteps"] / K,
+                      "lighting_hits": lc["hits"] / K, "draw_map_steps": dc["mapSteps"] / K,
+                      "draw_voxel_steps": dc["voxelSteps"] / K},
         "kernel_ms": {"lighting": light_ms, "draw": tm["drawMs"] / max(tm["drawLaunches"], 1),
                       "visible": tm["visibleMs"] / max(tm["visibleLaunches"], 1),
                       "epilogue": tm["epilogueMs"] / max(tm["epilogueLaunches"], 1)},

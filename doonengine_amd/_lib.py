@@ -6,6 +6,7 @@ with the library's own ``DNX_struct_size``.
 from __future__ import annotations
 
 import ctypes
+import os
 from ctypes import POINTER, Structure, c_bool, c_char_p, c_float, c_int, c_size_t, c_uint32, c_uint64, c_void_p
 from pathlib import Path
 
@@ -172,7 +173,9 @@ def load(path: Path | str | None = None) -> ctypes.CDLL:
     global _lib
     if _lib is not None and path is None:
         return _lib
-    p = Path(path) if path else LIB_PATH
+    # DOON_LIB selects another build of the library (e.g. a variant built with
+    # different kernel options) for side-by-side measurements.
+    p = Path(path) if path else Path(os.environ.get("DOON_LIB", str(LIB_PATH)))
     if not p.exists():
         raise RuntimeError(f"{p} is missing: build it with `python -m doonengine_amd.build`")
     lib = ctypes.CDLL(str(p))

@@ -48,27 +48,32 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"build step failed: {' '.join(cmd)}")
 
 
-def build(verbose: bool = False) -> Path:
-    """Compile every source and link libdoon.so. Always rebuilds."""
+def build(verbose: bool = False, defines: list[str] | None = None, out: Path | None = None) -> Path:
+    """Compile every source and link libdoon.so (or `out`). Always rebuilds.
+    `defines` adds -D options, for variant builds used in measurements."""
     if not Path(HIPCC).exists() and shutil.which("hipcc") is None:
         raise RuntimeError("hipcc not found")
-    BUILD.mkdir(exist_ok=True)
+    lib = Path(out) if out else LIB
+    build_dir = BUILD if out is None else BUILD / lib.stem
+    build_dir.mkdir(parents=True, exist_ok=True)
+    extra = [f"-D{d}" for d in (defines or [])]
     objs = []
     for name, is_hip in SOURCES:
         src = CSRC / name
-        obj = BUILD / (name + ".o")
-        cmd = [HIPCC] + COMMON + (DEVICE if is_hip else ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]) + ["-c", str(src), "-o", str(obj)]
+        obj = build_dir / (name + ".o")
+        cmd = [HIPCC] + COMMON + extra + (DEVICE if is_hip else ["-x", "c++", "-D__HIP_PLATFORM_AMD__",
+                                                                 "-I/opt/rocm/include"]) + ["-c", str(src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
         _run(cmd)
         objs.append(str(obj))
-    tmp = LIB.with_suffix(".so.tmp")
+    tmp = lib.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp)] + objs + ["-lpthread"]
     if verbose:
         print(" ".join(cmd), flush=True)
     _run(cmd)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":

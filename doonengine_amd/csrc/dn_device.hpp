@@ -315,6 +315,147 @@ __device__ bool step_map(const DNsceneView& s, RayCtx& c, f3& dir, f3 inv, f3& p
     return hit;
 }
 
+// step_map without refraction, as one loop over both DDA levels. The chunk
+// level and the map level share delta = |1/dir| and step = sign(dir), so the
+// trip ends with the same `lsd = side; iterate` for every lane whatever its
+// level, and only the occupancy test differs: lanes of a wave stay converged
+// instead of waiting on each other's inner chunk loops. The sequence of
+// floating-point operations per ray is exactly that of step_map<false>
+// (voxelShared.comp:328-475): map state is saved on chunk entry and resumed,
+// then stepped, when the chunk walk leaves the chunk.
+// With NORMAL, hitNormal receives the face normal of the last step taken (the
+// value iterate_dda leaves in it), computed once from the last step mask.
+template <bool NORMAL>
+__device__ __forceinline__ bool step_map_flat(const DNsceneView& s, RayCtx& c, f3 dir, f3 inv, f3& p,
+                                              bool ignoreFirst, DVoxel& vox, f3& colorAdd, float& colorMult,
+                                              f3& hitNormal)
+{
+    uint32_t lastMask = 0xFFFFFFFFu;
+    bool requested = false;
+    colorAdd = splat(0.0f);
+    colorMult = 1.0f;
+    uint32_t sMap = 0, sVox = 0;
+    c.hitIndex = 0xFFFFFFFFu;
+    DDA d;
+    init_dda(dir, inv, p, d);
+    f3 lsd = splat(0.0f);
+    const int32_t X = (int32_t)s.mapX, Y = (int32_t)s.mapY, Z = (int32_t)s.mapZ;
+    bool inChunk = false, chunkIgnore = false, hit = false;
+    int32_t mx = 0, my = 0, mz = 0;      // map cell while walking a chunk
+    f3 mside = splat(0.0f), crp0 = splat(0.0f);
+    uint32_t chunkIdx = 0;
+    Mask512 m;
+    const uint4* rq = nullptr;
+    for(;;)
+    {
+        const int32_t lx = inChunk ? 8 : X, ly = inChunk ? 8 : Y, lz = inChunk ? 8 : Z;
+        const bool inb = d.px >= 0 && d.py >= 0 && d.pz >= 0 && d.px < lx && d.py < ly && d.pz < lz;
+        if(!inb)
+        {
+            if(!inChunk) break;                              // left the map
+            d.px = mx; d.py = my; d.pz = mz;                 // chunk exhausted: resume the map walk
+            d.side = mside;
+            inChunk = false;
+        }
+        else if(!inChunk)
+        {
+            sMap++;
+            const uint32_t idx = (uint32_t)d.px + s.mapX * ((uint32_t)d.py + s.mapY * (uint32_t)d.pz);
+            if(occ_bit(s.occLoaded, idx))
+            {
+                if(s.lastUsed[idx] != 0u) s.lastUsed[idx] = 0u;
+                rq = reinterpret_cast<const uint4*>(s.chunks + idx);
+                m.q[0] = rq[0]; m.q[1] = rq[1]; m.q[2] = rq[2]; m.q[3] = rq[3];
+                f3 upd = p + dir * (minc(lsd) - DN_EPS);
+                f3 crp = (upd - mk((float)d.px, (float)d.py, (float)d.pz)) * 8.0f;
+                crp0 = vclamp(crp, DN_EPS, 8.0f - DN_EPS);
+                mx = d.px; my = d.py; mz = d.pz;
+                mside = d.side;
+                chunkIdx = idx;
+                // init_dda at chunk level: delta and step are unchanged
+                f3 f = vfloor(crp0);
+                d.px = (int32_t)f.x; d.py = (int32_t)f.y; d.pz = (int32_t)f.z;
+                f3 sg = mk((float)d.sx, (float)d.sy, (float)d.sz);
+                d.side = ((sg * (f - crp0) + sg * 0.5f) + 0.5f) * d.delta;
+                lsd = splat(0.0f);
+                chunkIgnore = ignoreFirst;
+                inChunk = true;
+                continue;                                    // first chunk cell on the next trip
+            }
+            else if(s.hasUnloaded && !requested && occ_bit(s.occPresent, idx))
+            {
+                s.flags[idx] = 3u;
+                requested = true;
+            }
+        }
+        else
+        {
+            sVox++;
+            const uint32_t li = (uint32_t)(d.px + 8 * (d.py + 8 * d.pz));
+            if(mask_bit(m, li) && !chunkIgnore)
+            {
+                uint4 tail = rq[4];
+                uint32_t partial[3] = {tail.x, tail.y, tail.z};
+                uint32_t vi = tail.w + rank_in(m, partial, li);
+                uint4 cw = s.voxels[vi];
+                vox = decompress(cw);
+                const DNmat& mat = s.mats[vox.material];
+                uint32_t thisID = (cw.y & 0xFFFFFF00u) | (cw.x >> 24);
+                if(mat.opacity == 1.0f)
+                {
+                    f3 crp = crp0 + dir * (minc(lsd) + DN_EPS);
+                    p = mk((float)mx, (float)my, (float)mz) + crp * 0.125f;
+                    c.hitIndex = vi;
+                    hit = true;
+                    break;
+                }
+                else if(c.lastVoxID != thisID)
+                {
+                    float op = mat.opacity;
+                    colorAdd = colorAdd + (vox.albedo * (colorMult * op)) * ld3(s.sunStrength);
+                    colorMult = colorMult * (1.0f - op);
+                    c.lastVoxID = thisID;
+                    c.lastVoxRefract = mat.refractIndex;
+                }
+            }
+            else if(c.lastVoxID != 255u)
+            {
+                c.lastVoxID = 255u;
+                c.lastVoxRefract = 1.0f;
+            }
+        }
+        // common step for both levels: lastSideDist = sideDist; iterate
+        lsd = d.side;
+        {
+            f3 sd = d.side;
+            bool bx = sd.x <= gmin(sd.y, sd.z);
+            bool by = sd.y <= gmin(sd.z, sd.x);
+            bool bz = sd.z <= gmin(sd.x, sd.y);
+            d.side.x = sd.x + (bx ? 1.0f : 0.0f) * d.delta.x;
+            d.side.y = sd.y + (by ? 1.0f : 0.0f) * d.delta.y;
+            d.side.z = sd.z + (bz ? 1.0f : 0.0f) * d.delta.z;
+            d.px += bx ? d.sx : 0; d.py += by ? d.sy : 0; d.pz += bz ? d.sz : 0;
+            if(NORMAL) lastMask = (bx ? 1u : 0u) | (by ? 2u : 0u) | (bz ? 4u : 0u);
+        }
+        if(inChunk) chunkIgnore = false;
+        else ignoreFirst = false;
+    }
+    (void)chunkIdx;
+    if(NORMAL && lastMask != 0xFFFFFFFFu)
+    {
+        hitNormal.x = ((lastMask & 1u) ? 1.0f : 0.0f) * (float)(-d.sx);
+        hitNormal.y = ((lastMask & 2u) ? 1.0f : 0.0f) * (float)(-d.sy);
+        hitNormal.z = ((lastMask & 4u) ? 1.0f : 0.0f) * (float)(-d.sz);
+    }
+    c.sMap = sMap;
+    c.sVox = sVox;
+    c.cRays++;
+    c.cMap += sMap;
+    c.cVox += sVox;
+    c.cHits += hit ? 1u : 0u;
+    return hit;
+}
+
 __device__ __forceinline__ f3 sky_color(const DNsceneView& s, f3 dir)
 {
     return vmix(ld3(s.skyBot), ld3(s.skyTop), dir.y * 0.5f + 1.0f);

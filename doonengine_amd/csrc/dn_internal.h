@@ -79,8 +79,13 @@ struct DNlightParams {
     float camPos[3];
     uint32_t numSpecRays, numDirectRays;
     const float* specPoints;       // device, numSpecRays * 3
-    const float* directPoints;     // device, numDirectRays * 3
     DNframeRandom rnd;
+    // Shadow-ray directions are the same for every voxel of a frame, so they
+    // come precomputed with their inverses (dir xyz, 1/dir xyz), in the exact
+    // FP32 operations of voxelLighting.comp:72-75:
+    float directRays[DN_MAX_POINTS][6];   // normalize(sunDir * softness + P_k) + eps, k < numDirectRays
+    float shadowRays[DN_MAX_SAMPLES][6];  // normalize(sunDir * softness + rand_unit_sphere(seed_i)) + eps
+    float firstShadow[6];                 // sunDir + eps (first sample)
 };
 
 struct DNdrawParams {
@@ -142,4 +147,4 @@ int dn_launch_upload(uint32_t* flags, uint32_t* lastUsed, DNchunkRec* chunks, ui
 int dn_launch_rays(const DNrayParams& p, hipStream_t stream);
 int dn_launch_collect_requested(const uint32_t* flags, uint32_t tiles, uint32_t* list, uint32_t* count,
                                 hipStream_t stream);
-int dn_launch_mark_visible(uint32_t* flags, uint32_t tiles, uint32_t lastTile, hipStream_t stream);
+int dn_launch_mark_visible(uint32_t* flags, const uint32_t* list, uint32_t n, hipStream_t stream);

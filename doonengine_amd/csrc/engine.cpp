@@ -161,6 +161,13 @@ struct DNgpuVolume {
     bool fullScan = false;
     bool requestAll = false;
     bool layoutDirty = true;
+    // exposed chunks (a solid voxel with an empty or see-through face neighbour),
+    // recomputed after edits, and the device list DNX_mark_visible uploads
+    std::vector<uint8_t> exposed;
+    bool exposedValid = false;
+    uint32_t* dMarkList = nullptr;
+    uint32_t markCount = 0, markMax = 0;
+    bool markValid = false;
     PoolAlloc pool;
 
     DNmaterial lastMats[DN_MAX_MATERIALS];
@@ -230,7 +237,9 @@ void free_tile_buffers(DNgpuVolume* g)
 {
     dfree(g->dFlags); dfree(g->dLastUsed); dfree(g->dChunks); dfree(g->dOccLoaded); dfree(g->dOccPresent);
     dfree(g->dFresh); dfree(g->dLitBits); dfree(g->dVisProp); dfree(g->dRequests); dfree(g->dBlockSums);
-    dfree(g->dReqList);
+    dfree(g->dReqList); dfree(g->dMarkList);
+    g->markValid = false;
+    g->exposedValid = false;
 }
 
 bool alloc_tile_buffers(DNgpuVolume* g, uint32_t tiles)
@@ -290,6 +299,8 @@ bool grow_pool(DNvolume* vol, uint32_t newCap)
 void mark_dirty(DNgpuVolume* g, uint32_t t)
 {
     if(t >= g->tiles) return;
+    g->exposedValid = false;
+    g->markValid = false;
     if(!g->dirtyMark[t]) { g->dirtyMark[t] = 1; g->dirty.push_back(t); }
 }
 
@@ -1165,8 +1176,24 @@ extern "C" void DN_update_lighting(DNvolume* vol, int numDiffuseSamples, int max
     p.numSpecRays = vol->numSpecularRays;
     p.numDirectRays = vol->numDirectRays;
     p.specPoints = g->dPoints;
-    p.directPoints = g->dPoints + DN_MAX_POINTS * 3;
     frame_random(vol->lastTime, (uint32_t)numDiffuseSamples, vol->diffuseBounceLimit, p.rnd);
+    // voxel-independent shadow rays, in the shader's FP32 operations (voxelLighting.comp:72-78)
+    {
+        auto ray = [](float x, float y, float z, float* o) {
+            o[0] = x; o[1] = y; o[2] = z;
+            o[3] = 1.0f / x; o[4] = 1.0f / y; o[5] = 1.0f / z;
+        };
+        auto soft = [&](const float* q, float* o) {
+            float vx = p.sunDir[0] * p.shadowSoftness + q[0];
+            float vy = p.sunDir[1] * p.shadowSoftness + q[1];
+            float vz = p.sunDir[2] * p.shadowSoftness + q[2];
+            float inv = 1.0f / sqrtf(vx * vx + vy * vy + vz * vz);
+            ray(vx * inv + 0.0001f, vy * inv + 0.0001f, vz * inv + 0.0001f, o);
+        };
+        for(uint32_t k = 0; k < vol->numDirectRays && k < DN_MAX_POINTS; k++) soft(&g->directPts[3 * k], p.directRays[k]);
+        for(int i = 0; i < numDiffuseSamples && i < DN_MAX_SAMPLES; i++) soft(p.rnd.shadowDir[i], p.shadowRays[i]);
+        ray(p.sunDir[0] + 0.0001f, p.sunDir[1] + 0.0001f, p.sunDir[2] + 0.0001f, p.firstShadow);
+    }
     uint64_t maxReq = std::min<uint64_t>(g->loadedBlocks, (uint64_t)g->tiles * 16u);
     Timer tm = timer_begin(g, TK_LIGHT);
     int e = dn_launch_lighting(p, (uint32_t)maxReq, g->stream);
@@ -1516,16 +1543,76 @@ extern "C" void DNX_set_voxels_dense(DNvolume* vol, DNivec3 origin, DNuvec3 size
 
 extern "C" void DNX_request_all_chunks(DNvolume* vol) { vol->gpu->requestAll = true; }
 
+namespace {
+// A voxel counts as see-through when it is empty, outside the map, or of a
+// material with opacity < 1 (the culling rule of voxel.c:1391-1394, across chunks).
+bool see_through(DNvolume* vol, int gx, int gy, int gz)
+{
+    if(gx < 0 || gy < 0 || gz < 0) return true;
+    int cx = gx >> 3, cy = gy >> 3, cz = gz >> 3;
+    if((uint32_t)cx >= vol->mapSize.x || (uint32_t)cy >= vol->mapSize.y || (uint32_t)cz >= vol->mapSize.z) return true;
+    const DNchunkHandle& h = vol->map[(uint32_t)cx + vol->mapSize.x * ((uint32_t)cy + vol->mapSize.y * (uint32_t)cz)];
+    if(h.flag == 0) return true;
+    uint32_t m = material_of(vol->chunks[h.chunkIndex].voxels[gx & 7][gy & 7][gz & 7].normal);
+    return m == DN_MATERIAL_EMPTY || vol->materials[m].opacity < 1.0f;
+}
+
+void compute_exposed(DNvolume* vol)
+{
+    DNgpuVolume* g = vol->gpu;
+    g->exposed.assign(g->tiles, 0);
+    unsigned nth = std::min<unsigned>(8, std::max<unsigned>(1, std::thread::hardware_concurrency()));
+    auto work = [&](unsigned w) {
+        for(uint32_t t = w; t < g->tiles; t += nth)
+        {
+            if(vol->map[t].flag == 0) continue;
+            const DNchunk& ch = vol->chunks[vol->map[t].chunkIndex];
+            int bx = (int)(t % vol->mapSize.x) * 8, by = (int)((t / vol->mapSize.x) % vol->mapSize.y) * 8,
+                bz = (int)(t / (vol->mapSize.x * vol->mapSize.y)) * 8;
+            bool any = false;
+            for(int x = 0; x < 8 && !any; x++)
+            for(int y = 0; y < 8 && !any; y++)
+            for(int z = 0; z < 8 && !any; z++)
+            {
+                if(material_of(ch.voxels[x][y][z].normal) == DN_MATERIAL_EMPTY) continue;
+                int gx = bx + x, gy = by + y, gz = bz + z;
+                any = see_through(vol, gx + 1, gy, gz) || see_through(vol, gx - 1, gy, gz) ||
+                      see_through(vol, gx, gy + 1, gz) || see_through(vol, gx, gy - 1, gz) ||
+                      see_through(vol, gx, gy, gz + 1) || see_through(vol, gx, gy, gz - 1);
+            }
+            g->exposed[t] = any ? 1 : 0;
+        }
+    };
+    std::vector<std::thread> th;
+    for(unsigned w = 0; w < nth; w++) th.emplace_back(work, w);
+    for(auto& x : th) x.join();
+    g->exposedValid = true;
+}
+}  // namespace
+
 extern "C" uint32_t DNX_mark_visible(DNvolume* vol, uint32_t maxChunks)
 {
     DNgpuVolume* g = vol->gpu;
-    uint32_t n = 0, last = 0;
-    for(uint32_t t = 0; t < g->tiles && n < maxChunks; t++)
-        if(g->state[t] == 2) { n++; last = t; }
-    if(n == 0) return 0;
-    int e = dn_launch_mark_visible(g->dFlags, g->tiles, last, g->stream);
+    if(!g->exposedValid) { compute_exposed(vol); g->markValid = false; }
+    if(!g->markValid || g->markMax != maxChunks)
+    {
+        std::vector<uint32_t> list;
+        for(uint32_t t = 0; t < g->tiles && list.size() < maxChunks; t++)
+            if(g->state[t] == 2 && g->exposed[t]) list.push_back(t);
+        if(!g->dMarkList && !dalloc(&g->dMarkList, g->tiles, "visible list")) return 0;
+        hipStreamSynchronize(g->stream);
+        if(!list.empty() &&
+           !HIP_OK(hipMemcpy(g->dMarkList, list.data(), sizeof(uint32_t) * list.size(), hipMemcpyHostToDevice),
+                   "visible list upload"))
+            return 0;
+        g->markCount = (uint32_t)list.size();
+        g->markMax = maxChunks;
+        g->markValid = true;
+    }
+    if(g->markCount == 0) return 0;
+    int e = dn_launch_mark_visible(g->dFlags, g->dMarkList, g->markCount, g->stream);
     if(e) dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR, "mark visible launch failed (%d)", e);
-    return n;
+    return g->markCount;
 }
 
 extern "C" void* DNX_get_stream(DNvolume* vol) { return (void*)vol->gpu->stream; }

@@ -251,9 +251,11 @@ void DNX_set_voxels_dense(DNvolume* vol, DNivec3 origin, DNuvec3 size,
  * DN_sync_gpu, instead of waiting for rays to request it. */
 void DNX_request_all_chunks(DNvolume* vol);
 
-/* Marks the first maxChunks loaded chunks (map-index order) visible, as primary
- * rays hitting them would (voxelDraw.comp:118-121): the next DN_sync_gpu requests
- * lighting for all of them. Returns the number of chunks marked. */
+/* Marks up to maxChunks exposed loaded chunks (map-index order) visible, as
+ * primary rays hitting them would (voxelDraw.comp:118-121): the next DN_sync_gpu
+ * requests lighting for all of them. A chunk is exposed when one of its voxels
+ * has an empty, out-of-map or see-through face neighbour (computed on the host,
+ * cached until the next edit). Returns the number of chunks marked. */
 uint32_t DNX_mark_visible(DNvolume* vol, uint32_t maxChunks);
 
 /* Device stream of the volume (a hipStream_t). */
