@@ -12,6 +12,7 @@ voxels (N, 4) u32, materials (256,) structured.
 from __future__ import annotations
 
 import ctypes
+import time as _time
 from ctypes import POINTER, Structure, c_float, c_int, c_uint32, c_uint64, c_void_p
 from pathlib import Path
 
@@ -160,9 +161,11 @@ class Oracle:
         px = np.zeros((height, width), PIXEL_DTYPE)
         cnt = ORcounters()
         fo = None if flags_out is None else flags_out.ctypes.data_as(c_void_p)
+        t0 = _time.perf_counter()
         self.lib.or_draw(ctypes.byref(scene), ctypes.byref(dp), rgba.ctypes.data_as(c_void_p),
                          px.ctypes.data_as(c_void_p), fo, ctypes.byref(cnt))
-        return rgba, px, {f: getattr(cnt, f) for f, _ in cnt._fields_}
+        dt = _time.perf_counter() - t0
+        return rgba, px, dict({f: getattr(cnt, f) for f, _ in cnt._fields_}, seconds=dt)
 
     def lighting(self, scene: ORscene, requests: np.ndarray, *, time: float, num_diffuse=1, max_diffuse=1000,
                  diffuse_bounces=5, spec_bounces=2, sun_dir=(1, 1, 1), ambient=(0.01, 0.01, 0.01), softness=10.0,
@@ -194,12 +197,14 @@ class Oracle:
         preq = np.zeros((R * 32, 6), self.real)
         vox_of = np.zeros(R * 32, np.uint32)
         cnt = ORcounters()
+        t0 = _time.perf_counter()
         self.lib.or_lighting(ctypes.byref(s), ctypes.byref(lp), req.ctypes.data_as(c_void_p), c_uint32(len(req)),
                              v_out.ctypes.data_as(c_void_p), m_out.ctypes.data_as(c_void_p),
                              c_out.ctypes.data_as(c_void_p), preq.ctypes.data_as(c_void_p),
                              vox_of.ctypes.data_as(c_void_p), ctypes.byref(cnt))
-        return v_out, m_out, c_out, preq[:len(req) * 32], vox_of[:len(req) * 32], {
-            f: getattr(cnt, f) for f, _ in cnt._fields_}
+        dt = _time.perf_counter() - t0
+        return v_out, m_out, c_out, preq[:len(req) * 32], vox_of[:len(req) * 32], dict(
+            {f: getattr(cnt, f) for f, _ in cnt._fields_}, seconds=dt)
 
 
 _cache: dict = {}
