@@ -107,6 +107,17 @@ class DNXpixelHit(Structure):
                 ("face", c_float * 3)]
 
 
+class DNXbatchConfig(Structure):
+    _fields_ = [("requestsPerUnit", c_uint32), ("blockThreads", c_uint32), ("blocksPerCU", c_uint32),
+                ("xcdQueues", c_uint32)]
+
+
+class DNXrcConfig(Structure):
+    _fields_ = [("log2Table", c_uint32), ("nMin", c_uint32), ("nMax", c_uint32), ("extent", c_float),
+                ("lr", c_float), ("beta1", c_float), ("beta2", c_float), ("eps", c_float),
+                ("samplesPerFrame", c_uint32), ("seed", c_uint32)]
+
+
 MSG_CB = ctypes.CFUNCTYPE(None, c_int, c_int, c_char_p)
 
 # name: (restype, argtypes)
@@ -159,11 +170,31 @@ SIGNATURES = {
     "DNX_cast_rays": (None, [POINTER(DNvolume), c_uint32, c_void_p, c_void_p, c_bool, c_void_p]),
     "DNX_set_draw_hit_output": (None, [POINTER(DNvolume), c_void_p, c_size_t]),
     "DNX_set_lighting_debug_output": (None, [POINTER(DNvolume), c_void_p, c_void_p, c_size_t]),
+    "DNX_set_lighting_sample_output": (None, [POINTER(DNvolume), c_void_p, c_size_t]),
     "DNX_draw_matrices": (None, [DNmat4, DNmat4, POINTER(DNmat4), POINTER(DNmat4)]),
+    "DNX_batch_create": (c_void_p, [POINTER(POINTER(DNvolume)), c_uint32]),
+    "DNX_batch_destroy": (None, [c_void_p]),
+    "DNX_batch_set_config": (c_bool, [c_void_p, POINTER(DNXbatchConfig)]),
+    "DNX_batch_get_config": (None, [c_void_p, POINTER(DNXbatchConfig)]),
+    "DNX_batch_update_lighting": (c_bool, [c_void_p, c_int, c_int, c_float]),
+    "DNX_batch_autotune": (c_bool, [c_void_p, c_int, c_int, c_float, c_int, POINTER(DNXbatchConfig),
+                                    POINTER(c_float)]),
+    "DNX_batch_timing": (ctypes.c_double, [c_void_p, POINTER(c_uint32), c_bool]),
+    "DNX_rc_default_config": (None, [POINTER(DNXrcConfig)]),
+    "DNX_rc_create": (c_void_p, [POINTER(DNvolume), POINTER(DNXrcConfig)]),
+    "DNX_rc_destroy": (None, [c_void_p]),
+    "DNX_rc_train_frame": (c_bool, [c_void_p, c_float, c_int]),
+    "DNX_rc_predict_requests": (c_bool, [c_void_p, c_void_p, c_size_t]),
+    "DNX_rc_train_samples": (c_bool, [c_void_p, c_void_p, c_uint32, c_bool]),
+    "DNX_rc_predict_samples": (c_bool, [c_void_p, c_void_p, c_uint32, c_void_p]),
+    "DNX_rc_set_params": (c_bool, [c_void_p, c_void_p, c_void_p]),
+    "DNX_rc_get_params": (c_bool, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "DNX_rc_last_samples": (c_void_p, [c_void_p, POINTER(c_uint32)]),
+    "DNX_accumulate_f64": (c_bool, [POINTER(DNvolume), c_void_p, c_void_p, c_size_t]),
 }
 
 STRUCT_IDS = {0: DNvolume, 1: DNchunk, 2: DNmaterial, 3: DNchunkHandle, 4: DNvoxelNode, 5: DNXcounters,
-              6: DNXtiming, 7: DNXrayHit, 8: DNXpixelHit, 9: DNvoxel}
+              6: DNXtiming, 7: DNXrayHit, 8: DNXpixelHit, 9: DNvoxel, 12: DNXbatchConfig, 13: DNXrcConfig}
 
 _lib = None
 

@@ -36,7 +36,9 @@ DEVICE = [
 
 SOURCES = [
     ("engine.cpp", False),
+    ("rcache.cpp", False),
     ("kernels.hip", True),
+    ("rcache.hip", True),
 ]
 
 

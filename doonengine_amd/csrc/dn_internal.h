@@ -70,7 +70,10 @@ struct DNlightParams {
     uint4* voxelsOut;              // pool receiving lit words (== voxels when written in place)
     float* debugValues;            // optional: 6 floats per slot
     uint32_t* debugIndex;          // optional: pool index per slot
+    float* debugFresh;             // optional: 4 floats per slot, this pass's diffuse samples
+                                   // averaged before the running mean (w = 1 where written)
     uint32_t debugSlots;           // capacity of the debug outputs in slots
+    uint32_t freshSlots;           // capacity of debugFresh in slots
     uint64_t* counters;            // 8 x u64
     uint32_t numDiffuseSamples, maxDiffuseSamples, diffuseBounceLimit, specBounceLimit;
     float sunDir[3];
@@ -86,6 +89,25 @@ struct DNlightParams {
     float directRays[DN_MAX_POINTS][6];   // normalize(sunDir * softness + P_k) + eps, k < numDirectRays
     float shadowRays[DN_MAX_SAMPLES][6];  // normalize(sunDir * softness + rand_unit_sphere(seed_i)) + eps
     float firstShadow[6];                 // sunDir + eps (first sample)
+    uint32_t tiles;                       // map tiles (bitmap words = ceil(tiles / 32))
+    uint32_t copyBack;                    // voxelsOut != voxels: copy lit voxels back after the pass
+};
+
+// Batched lighting over many volumes in one persistent launch. Work units are
+// `unit` consecutive requests of one volume; volumes are dealt to `queues`
+// queues (1, or 8 = one per XCD), each with its own unit counter; a wave drains
+// its own XCD's queue first, then the others.
+enum { DN_BATCH_MAX_QUEUES = 8 };
+struct DNbatchParams {
+    const DNlightParams* scenes;   // device, one per volume
+    uint32_t numScenes;
+    uint32_t unit;                 // requests per work unit, even
+    uint32_t queues;               // 1 or 8
+    uint32_t dry;                  // 1: compute only, no stores (autotuning)
+    uint32_t* queueCounters;       // [queues] next unit of each queue
+    const uint32_t* queueScenes;   // [queues][numScenes] volume ids of each queue, in order
+    const uint32_t* queueLen;      // [queues] volumes in each queue
+    uint32_t* queuePrefix;         // [queues][numScenes + 1] exclusive unit prefix, written on device
 };
 
 struct DNdrawParams {
@@ -134,6 +156,63 @@ struct DNvisibleParams {
 // Counter slots (u64 each) per kind.
 enum { CNT_RAYS = 0, CNT_MAPSTEPS, CNT_VOXSTEPS, CNT_HITS, CNT_LIT, CNT_PIXELS, CNT_SLOTS = 8 };
 
+// Radiance cache network: multiresolution hash encoding (RC_LEVELS levels of
+// RC_FEAT features) plus an MLP of four weight matrices (3 hidden layers of
+// RC_W with ReLU, linear output of 3), no biases. Weights are stored row-major
+// [out][in] and zero-padded: W0 64x48 (38 inputs used), W1 64x64, W2 64x64,
+// W3 16x64 (3 outputs used). A sample row is 16 floats: position (voxel
+// units) 3, normal 3, albedo 3, target 3, valid (0/1), 3 unused.
+enum {
+    RC_LEVELS = 16, RC_FEAT = 2, RC_ENC = 32, RC_IN = 48, RC_W = 64, RC_OUT = 16, RC_SAMPLE = 16,
+    RC_OFF_W0 = 0, RC_OFF_W1 = 3072, RC_OFF_W2 = 7168, RC_OFF_W3 = 11264, RC_NPARAM = 12288
+};
+struct DNrcNet {
+    uint32_t log2T;
+    uint32_t denseMask;            // bit l: level l indexes densely ((N_l + 2)^3 <= T)
+    uint32_t res[RC_LEVELS];       // N_l
+    float extent;                  // positions / extent in [0, 1)
+    uint32_t* tab16;               // [L][T] two bf16 features per entry
+    float* tab32;                  // [L][T][2] master copy
+    float* tabGrad;
+    float* tabM;
+    float* tabV;
+    uint16_t* w16;                 // [RC_NPARAM] bf16
+    float* w32;                    // master copy
+    float* wGrad;
+    float* wM;
+    float* wV;
+};
+// Per-batch activation buffers of training, `tiles` = samples / 16. The *T
+// buffers are tile-transposed bf16: [tile][feature][16 samples].
+struct DNrcActs {
+    uint16_t* x0;                  // [samples][48] bf16, sample-major
+    uint16_t* x0T;                 // [tile][48][16]
+    uint16_t* hT[3];               // [tile][64][16]
+    uint16_t* dzT[3];              // [tile][64][16]
+    uint16_t* dyT;                 // [tile][16][16]
+    float* denc;                   // [samples][32]
+};
+struct DNrcAdam {
+    float lr, beta1, beta2, eps;
+    float bias1, bias2;            // 1 - beta^t
+};
+
+int dn_rc_encode(const DNrcNet& net, const float* samples, uint32_t n, uint16_t* x0, hipStream_t stream);
+// pred: float4 per row (3 outputs, the row's valid flag; 1 without samples).
+int dn_rc_forward(const DNrcNet& net, const uint16_t* x0, const float* samples, uint32_t n, float* pred,
+                  hipStream_t stream);
+int dn_rc_train(const DNrcNet& net, const float* samples, uint32_t n, const DNrcActs& a, hipStream_t stream);
+int dn_rc_adam(const DNrcNet& net, const DNrcAdam& opt, hipStream_t stream);
+int dn_rc_samples(const DNlightParams& p, uint32_t maxRequests, uint32_t n, uint32_t seed, float* samples,
+                  hipStream_t stream);
+int dn_rc_gather(const DNlightParams& p, uint32_t slots, float* samples, hipStream_t stream);
+int dn_accumulate_f64(const float* src, double* dst, size_t n, hipStream_t stream);
+
+// Fills the lighting-pass parameters of `vol` for a frame (engine.cpp).
+struct DNvolume;
+bool prepare_lighting(DNvolume* vol, int numDiffuseSamples, int maxDiffuseSamples, float time, DNlightParams& p,
+                      uint32_t& maxRequests);
+
 // Kernel launchers (kernels.hip). All asynchronous on `stream`; they return a
 // hipError_t value.
 int dn_launch_draw(const DNdrawParams& p, hipStream_t stream);
@@ -148,3 +227,8 @@ int dn_launch_rays(const DNrayParams& p, hipStream_t stream);
 int dn_launch_collect_requested(const uint32_t* flags, uint32_t tiles, uint32_t* list, uint32_t* count,
                                 hipStream_t stream);
 int dn_launch_mark_visible(uint32_t* flags, const uint32_t* list, uint32_t n, hipStream_t stream);
+// Batched pass: unit prefix + counter reset, persistent lighting kernel, then
+// the per-volume epilogue and copy-back. `blockThreads` 64 or 256, `blocksPerCU`
+// 0 = as many as fit. `maxTiles` / `maxRequests`: largest over the volumes.
+int dn_launch_batch(const DNbatchParams& b, uint32_t blockThreads, uint32_t blocksPerCU, uint32_t maxTiles,
+                    uint32_t maxRequests, bool epilogue, hipStream_t stream);

@@ -20,9 +20,11 @@
 void (*g_DN_message_callback)(DNmessageType, DNmessageSeverity, const char*) = nullptr;
 
 namespace {
-
 std::atomic<uint32_t> g_errors{0};
+}  // namespace
 
+// Message dispatch through g_DN_message_callback (voxel.c:102); errors are
+// also counted. Used by every host source of the library.
 void dn_msg(DNmessageType type, DNmessageSeverity sev, const char* fmt, ...)
 {
     char buf[512];
@@ -36,6 +38,8 @@ void dn_msg(DNmessageType type, DNmessageSeverity sev, const char* fmt, ...)
     else if(sev != DN_MESSAGE_NOTE)
         fprintf(stderr, "doon: %s\n", buf);
 }
+
+namespace {
 
 #define HIP_OK(expr, what)                                                                                  \
     ([&]() -> bool {                                                                                        \
@@ -185,6 +189,8 @@ struct DNgpuVolume {
     float* dbgValues = nullptr;
     uint32_t* dbgIndex = nullptr;
     size_t dbgCap = 0;
+    float* freshOut = nullptr;
+    size_t freshCap = 0;
 };
 
 namespace {
@@ -280,8 +286,10 @@ bool grow_pool(DNvolume* vol, uint32_t newCap)
     if(newCap <= g->pool.cap) return true;
     uint4* nv = nullptr;
     if(!dalloc(&nv, (size_t)newCap, "voxel buffer")) return false;
-    if(g->dVoxels && g->pool.cap)
-        hipMemcpyAsync(nv, g->dVoxels, sizeof(uint4) * g->pool.cap, hipMemcpyDeviceToDevice, g->stream);
+    const size_t old = g->dVoxels ? g->pool.cap : 0;
+    if(old) hipMemcpyAsync(nv, g->dVoxels, sizeof(uint4) * old, hipMemcpyDeviceToDevice, g->stream);
+    // free slots read as zero words, so exported pools are deterministic
+    hipMemsetAsync(nv + old, 0, sizeof(uint4) * (newCap - old), g->stream);
     hipStreamSynchronize(g->stream);
     dfree(g->dVoxels);
     g->dVoxels = nv;
@@ -1136,7 +1144,11 @@ extern "C" void DN_draw(DNvolume* vol, DNtexture out, DNmat4 view, DNmat4 projec
 // ===========================================================================
 // LIGHTING (voxel.c:883-952)
 
-extern "C" void DN_update_lighting(DNvolume* vol, int numDiffuseSamples, int maxDiffuseSamples, float time)
+// Everything DN_update_lighting does before its launches (voxel.c:883-945):
+// validates the limits, uploads materials and ray tables, and fills the kernel
+// parameters. Shared by the single-volume and the batched lighting pass.
+bool prepare_lighting(DNvolume* vol, int numDiffuseSamples, int maxDiffuseSamples, float time, DNlightParams& p,
+                      uint32_t& maxRequests)
 {
     DNgpuVolume* g = vol->gpu;
     if(vol->frameNum == 0) vol->lastTime = time;
@@ -1145,13 +1157,14 @@ extern "C" void DN_update_lighting(DNvolume* vol, int numDiffuseSamples, int max
     {
         dn_msg(DN_MESSAGE_SHADER, DN_MESSAGE_ERROR,
                "lighting limits: numDiffuseSamples in [0, %d], diffuseBounceLimit <= %d", DN_MAX_SAMPLES, DN_MAX_BOUNCES);
-        return;
+        return false;
     }
-    if(!upload_materials(vol) || !ensure_points(vol)) return;
+    if(!upload_materials(vol) || !ensure_points(vol)) return false;
     if(g->anySpecular && !g->dVoxelsOut)
-        if(!dalloc(&g->dVoxelsOut, (size_t)g->pool.cap, "voxel output buffer")) return;
-    DNlightParams p;
+        if(!dalloc(&g->dVoxelsOut, (size_t)g->pool.cap, "voxel output buffer")) return false;
     memset(&p, 0, sizeof(p));
+    p.tiles = g->tiles;
+    p.copyBack = g->anySpecular ? 1u : 0u;
     p.scene = scene_view(vol);
     p.requests = g->dRequests;
     p.numRequests = g->dNumRequests;
@@ -1162,6 +1175,8 @@ extern "C" void DN_update_lighting(DNvolume* vol, int numDiffuseSamples, int max
     p.debugValues = g->dbgValues;
     p.debugIndex = g->dbgIndex;
     p.debugSlots = (uint32_t)std::min<size_t>(g->dbgCap * 32u, 0xFFFFFFFFu);
+    p.debugFresh = g->freshOut;
+    p.freshSlots = (uint32_t)std::min<size_t>(g->freshCap * 32u, 0xFFFFFFFFu);
     p.counters = g->dCounters + 1 * CNT_SLOTS;
     p.numDiffuseSamples = (uint32_t)numDiffuseSamples;
     p.maxDiffuseSamples = (uint32_t)maxDiffuseSamples;
@@ -1194,9 +1209,18 @@ extern "C" void DN_update_lighting(DNvolume* vol, int numDiffuseSamples, int max
         for(int i = 0; i < numDiffuseSamples && i < DN_MAX_SAMPLES; i++) soft(p.rnd.shadowDir[i], p.shadowRays[i]);
         ray(p.sunDir[0] + 0.0001f, p.sunDir[1] + 0.0001f, p.sunDir[2] + 0.0001f, p.firstShadow);
     }
-    uint64_t maxReq = std::min<uint64_t>(g->loadedBlocks, (uint64_t)g->tiles * 16u);
+    maxRequests = (uint32_t)std::min<uint64_t>(g->loadedBlocks, (uint64_t)g->tiles * 16u);
+    return true;
+}
+
+extern "C" void DN_update_lighting(DNvolume* vol, int numDiffuseSamples, int maxDiffuseSamples, float time)
+{
+    DNgpuVolume* g = vol->gpu;
+    DNlightParams p;
+    uint32_t maxReq = 0;
+    if(!prepare_lighting(vol, numDiffuseSamples, maxDiffuseSamples, time, p, maxReq)) return;
     Timer tm = timer_begin(g, TK_LIGHT);
-    int e = dn_launch_lighting(p, (uint32_t)maxReq, g->stream);
+    int e = dn_launch_lighting(p, maxReq, g->stream);
     timer_end(g, tm);
     Timer te = timer_begin(g, TK_EPILOGUE);
     int e2 = dn_launch_lighting_epilogue(p, g->tiles, g->dRequests, g->dNumRequests, g->anySpecular ? 1 : 0, g->stream);
@@ -1755,6 +1779,12 @@ extern "C" void DNX_set_lighting_debug_output(DNvolume* vol, float* values, uint
     vol->gpu->dbgCap = requestCapacity;
 }
 
+extern "C" void DNX_set_lighting_sample_output(DNvolume* vol, float* samples, size_t requestCapacity)
+{
+    vol->gpu->freshOut = samples;
+    vol->gpu->freshCap = samples ? requestCapacity : 0;
+}
+
 // Sizes of the public structs, so bindings can check their mirrors.
 extern "C" size_t DNX_struct_size(int which)
 {
@@ -1772,6 +1802,309 @@ extern "C" size_t DNX_struct_size(int which)
     case 9: return sizeof(DNvoxel);
     case 10: return offsetof(DNvolume, numSpecularRays);
     case 11: return offsetof(DNvolume, camPos);
+    case 12: return sizeof(DNXbatchConfig);
+    case 13: return sizeof(DNXrcConfig);
     default: return 0;
     }
+}
+
+// ===========================================================================
+// BATCHED LIGHTING (no counterpart in the reference)
+
+struct DNXbatch {
+    std::vector<DNvolume*> vols;
+    hipStream_t stream = nullptr;
+    DNXbatchConfig cfg = {8, 256, 0, 1};
+    DNlightParams* hParams[2] = {nullptr, nullptr};   // pinned, one per frame slot
+    DNlightParams* dParams = nullptr;
+    uint32_t* dQueueCounters = nullptr;
+    uint32_t* dQueueScenes = nullptr;                 // [8][n]
+    uint32_t* dQueueLen = nullptr;                    // [8]
+    uint32_t* dQueuePrefix = nullptr;                 // [8][n + 1]
+    uint32_t queues = 0;                              // queue tables uploaded for this many queues
+    hipEvent_t ready = nullptr;                       // re-recorded on each volume's stream
+    hipEvent_t slotDone[2] = {nullptr, nullptr};
+    bool slotUsed[2] = {false, false};
+    int slot = 0;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    DNXbatchConfig execCfg[2];
+    uint32_t execTiles[2] = {0, 0}, execReq[2] = {0, 0};
+    bool timing = true;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, spare;
+    double ms = 0.0;
+    uint32_t launches = 0;
+};
+
+namespace {
+
+bool batch_config_ok(const DNXbatchConfig& c)
+{
+    return c.requestsPerUnit >= 2u && c.requestsPerUnit <= 64u && !(c.requestsPerUnit & 1u) &&
+           (c.blockThreads == 64u || c.blockThreads == 256u) && c.blocksPerCU <= 64u && c.xcdQueues <= 1u;
+}
+
+// Deals the volumes to the queues (volume i -> queue i % Q, in order) and
+// uploads the tables; synchronous, outside any capture.
+bool batch_upload_queues(DNXbatch* b)
+{
+    const uint32_t n = (uint32_t)b->vols.size();
+    const uint32_t Q = b->cfg.xcdQueues ? (uint32_t)DN_BATCH_MAX_QUEUES : 1u;
+    if(b->queues == Q) return true;
+    hipStreamSynchronize(b->stream);   // an earlier launch may still read the tables
+    std::vector<uint32_t> ids((size_t)DN_BATCH_MAX_QUEUES * n, 0u), len(DN_BATCH_MAX_QUEUES, 0u);
+    for(uint32_t i = 0; i < n; i++)
+    {
+        uint32_t q = i % Q;
+        ids[(size_t)q * n + len[q]++] = i;
+    }
+    if(!HIP_OK(hipMemcpy(b->dQueueScenes, ids.data(), sizeof(uint32_t) * ids.size(), hipMemcpyHostToDevice),
+               "batch queue table") ||
+       !HIP_OK(hipMemcpy(b->dQueueLen, len.data(), sizeof(uint32_t) * len.size(), hipMemcpyHostToDevice),
+               "batch queue table"))
+        return false;
+    b->queues = Q;
+    return true;
+}
+
+DNbatchParams batch_params(DNXbatch* b, bool dry)
+{
+    DNbatchParams bp;
+    bp.scenes = b->dParams;
+    bp.numScenes = (uint32_t)b->vols.size();
+    bp.unit = b->cfg.requestsPerUnit;
+    bp.queues = b->queues;
+    bp.dry = dry ? 1u : 0u;
+    bp.queueCounters = b->dQueueCounters;
+    bp.queueScenes = b->dQueueScenes;
+    bp.queueLen = b->dQueueLen;
+    bp.queuePrefix = b->dQueuePrefix;
+    return bp;
+}
+
+// Fills the parameter slot for every volume and orders the batch stream after
+// each volume's stream.
+bool batch_prepare(DNXbatch* b, DNlightParams* dst, int nds, int mds, float time, uint32_t& maxTiles,
+                   uint32_t& maxReq)
+{
+    maxTiles = 0;
+    maxReq = 0;
+    for(size_t i = 0; i < b->vols.size(); i++)
+    {
+        DNvolume* vol = b->vols[i];
+        uint32_t mr = 0;
+        if(!prepare_lighting(vol, nds, mds, time, dst[i], mr)) return false;
+        maxTiles = std::max(maxTiles, vol->gpu->tiles);
+        maxReq = std::max(maxReq, mr);
+        hipEventRecord(b->ready, vol->gpu->stream);
+        hipStreamWaitEvent(b->stream, b->ready, 0);
+    }
+    return true;
+}
+
+void batch_free_execs(DNXbatch* b)
+{
+    for(int s = 0; s < 2; s++)
+        if(b->exec[s])
+        {
+            hipGraphExecDestroy(b->exec[s]);
+            b->exec[s] = nullptr;
+        }
+}
+
+}  // namespace
+
+extern "C" DNXbatch* DNX_batch_create(DNvolume* const* volumes, uint32_t count)
+{
+    if(!volumes || count == 0 || count > 65535u)
+    {
+        dn_msg(DN_MESSAGE_CPU_MEMORY, DN_MESSAGE_ERROR, "batch: 1..65535 volumes required");
+        return nullptr;
+    }
+    for(uint32_t i = 0; i < count; i++)
+        if(!volumes[i] || !volumes[i]->gpu)
+        {
+            dn_msg(DN_MESSAGE_CPU_MEMORY, DN_MESSAGE_ERROR, "batch: volume %u is not a live volume", i);
+            return nullptr;
+        }
+    DNXbatch* b = new DNXbatch();
+    b->vols.assign(volumes, volumes + count);
+    bool ok = HIP_OK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "batch stream") &&
+              HIP_OK(hipEventCreateWithFlags(&b->ready, hipEventDisableTiming), "batch event");
+    for(int s = 0; ok && s < 2; s++)
+        ok = HIP_OK(hipHostMalloc((void**)&b->hParams[s], sizeof(DNlightParams) * count, hipHostMallocDefault),
+                    "batch parameters") &&
+             HIP_OK(hipEventCreateWithFlags(&b->slotDone[s], hipEventDisableTiming), "batch event");
+    ok = ok && dalloc(&b->dParams, count, "batch parameters") &&
+         dalloc(&b->dQueueCounters, DN_BATCH_MAX_QUEUES, "batch queues") &&
+         dalloc(&b->dQueueScenes, (size_t)DN_BATCH_MAX_QUEUES * count, "batch queues") &&
+         dalloc(&b->dQueueLen, DN_BATCH_MAX_QUEUES, "batch queues") &&
+         dalloc(&b->dQueuePrefix, (size_t)DN_BATCH_MAX_QUEUES * (count + 1u), "batch queues");
+    if(!ok)
+    {
+        DNX_batch_destroy(b);
+        return nullptr;
+    }
+    return b;
+}
+
+extern "C" void DNX_batch_destroy(DNXbatch* b)
+{
+    if(!b) return;
+    if(b->stream) hipStreamSynchronize(b->stream);
+    batch_free_execs(b);
+    for(auto& t : b->pending) { hipEventDestroy(t.first); hipEventDestroy(t.second); }
+    for(auto& t : b->spare) { hipEventDestroy(t.first); hipEventDestroy(t.second); }
+    for(int s = 0; s < 2; s++)
+    {
+        if(b->hParams[s]) hipHostFree(b->hParams[s]);
+        if(b->slotDone[s]) hipEventDestroy(b->slotDone[s]);
+    }
+    dfree(b->dParams); dfree(b->dQueueCounters); dfree(b->dQueueScenes); dfree(b->dQueueLen); dfree(b->dQueuePrefix);
+    if(b->ready) hipEventDestroy(b->ready);
+    if(b->stream) hipStreamDestroy(b->stream);
+    delete b;
+}
+
+extern "C" bool DNX_batch_set_config(DNXbatch* b, const DNXbatchConfig* c)
+{
+    if(!b || !c || !batch_config_ok(*c))
+    {
+        dn_msg(DN_MESSAGE_SHADER, DN_MESSAGE_ERROR,
+               "batch config: requestsPerUnit even in [2, 64], blockThreads 64 or 256, blocksPerCU <= 64, "
+               "xcdQueues 0 or 1");
+        return false;
+    }
+    b->cfg = *c;
+    return true;
+}
+
+extern "C" void DNX_batch_get_config(DNXbatch* b, DNXbatchConfig* c)
+{
+    if(b && c) *c = b->cfg;
+}
+
+extern "C" bool DNX_batch_update_lighting(DNXbatch* b, int numDiffuseSamples, int maxDiffuseSamples, float time)
+{
+    if(!b) return false;
+    const int s = b->slot;
+    if(b->slotUsed[s]) hipEventSynchronize(b->slotDone[s]);   // the slot's parameters were consumed
+    if(!batch_upload_queues(b)) return false;
+    uint32_t maxTiles = 0, maxReq = 0;
+    if(!batch_prepare(b, b->hParams[s], numDiffuseSamples, maxDiffuseSamples, time, maxTiles, maxReq)) return false;
+    const DNXbatchConfig& c = b->cfg;
+    const DNXbatchConfig& e = b->execCfg[s];
+    bool stale = !b->exec[s] || e.requestsPerUnit != c.requestsPerUnit || e.blockThreads != c.blockThreads ||
+                 e.blocksPerCU != c.blocksPerCU || e.xcdQueues != c.xcdQueues || b->execTiles[s] != maxTiles ||
+                 b->execReq[s] != maxReq;
+    if(stale)
+    {
+        if(b->exec[s]) { hipGraphExecDestroy(b->exec[s]); b->exec[s] = nullptr; }
+        // capture on a private stream so the waits on the volume streams stay outside the graph
+        hipStream_t cs = nullptr;
+        if(!HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "batch capture stream")) return false;
+        hipGraph_t graph = nullptr;
+        bool ok = HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed), "batch capture");
+        if(ok)
+        {
+            hipMemcpyAsync(b->dParams, b->hParams[s], sizeof(DNlightParams) * b->vols.size(), hipMemcpyHostToDevice,
+                           cs);
+            int le = dn_launch_batch(batch_params(b, false), c.blockThreads, c.blocksPerCU, maxTiles, maxReq, true, cs);
+            ok = HIP_OK(hipStreamEndCapture(cs, &graph), "batch capture") && le == 0;
+            if(ok) ok = HIP_OK(hipGraphInstantiate(&b->exec[s], graph, nullptr, nullptr, 0), "batch graph");
+            if(graph) hipGraphDestroy(graph);
+        }
+        hipStreamDestroy(cs);
+        if(!ok) return false;
+        b->execCfg[s] = c;
+        b->execTiles[s] = maxTiles;
+        b->execReq[s] = maxReq;
+    }
+    std::pair<hipEvent_t, hipEvent_t> t(nullptr, nullptr);
+    if(b->timing)
+    {
+        if(!b->spare.empty()) { t = b->spare.back(); b->spare.pop_back(); }
+        else { hipEventCreate(&t.first); hipEventCreate(&t.second); }
+        hipEventRecord(t.first, b->stream);
+    }
+    bool ok = HIP_OK(hipGraphLaunch(b->exec[s], b->stream), "batch launch");
+    if(t.first)
+    {
+        hipEventRecord(t.second, b->stream);
+        b->pending.push_back(t);
+    }
+    hipEventRecord(b->slotDone[s], b->stream);
+    b->slotUsed[s] = true;
+    for(DNvolume* vol : b->vols) hipStreamWaitEvent(vol->gpu->stream, b->slotDone[s], 0);
+    b->slot = s ^ 1;
+    return ok;
+}
+
+extern "C" double DNX_batch_timing(DNXbatch* b, uint32_t* launches, bool reset)
+{
+    if(!b) return 0.0;
+    for(auto& t : b->pending)
+    {
+        hipEventSynchronize(t.second);
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, t.first, t.second);
+        b->ms += ms;
+        b->launches++;
+        b->spare.push_back(t);
+    }
+    b->pending.clear();
+    double ms = b->ms;
+    if(launches) *launches = b->launches;
+    if(reset) { b->ms = 0.0; b->launches = 0; }
+    return ms;
+}
+
+extern "C" bool DNX_batch_autotune(DNXbatch* b, int numDiffuseSamples, int maxDiffuseSamples, float time, int repeats,
+                                   DNXbatchConfig* best, float* bestMs)
+{
+    if(!b) return false;
+    if(repeats < 1) repeats = 1;
+    for(int s = 0; s < 2; s++)
+        if(b->slotUsed[s]) hipEventSynchronize(b->slotDone[s]);
+    uint32_t maxTiles = 0, maxReq = 0;
+    DNlightParams* hp = b->hParams[b->slot];
+    if(!batch_prepare(b, hp, numDiffuseSamples, maxDiffuseSamples, time, maxTiles, maxReq)) return false;
+    if(!HIP_OK(hipMemcpyAsync(b->dParams, hp, sizeof(DNlightParams) * b->vols.size(), hipMemcpyHostToDevice,
+                              b->stream), "batch parameters"))
+        return false;
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    const DNXbatchConfig keep = b->cfg;
+    DNXbatchConfig win = keep;
+    float winMs = -1.0f;
+    const uint32_t units[] = {2, 4, 8, 16, 32};
+    const uint32_t threads[] = {64, 256};
+    bool ok = true;
+    for(uint32_t xq = 0; xq < 2 && ok; xq++)
+        for(uint32_t th : threads)
+            for(uint32_t u : units)
+            {
+                b->cfg = DNXbatchConfig{u, th, 0, xq};
+                if(!batch_upload_queues(b)) { ok = false; break; }
+                float ms = 0.0f, mn = -1.0f;
+                for(int r = 0; r < repeats; r++)
+                {
+                    hipEventRecord(e0, b->stream);
+                    dn_launch_batch(batch_params(b, true), th, 0, maxTiles, maxReq, false, b->stream);
+                    hipEventRecord(e1, b->stream);
+                    hipEventSynchronize(e1);
+                    hipEventElapsedTime(&ms, e0, e1);
+                    if(mn < 0.0f || ms < mn) mn = ms;
+                }
+                if(winMs < 0.0f || mn < winMs) { winMs = mn; win = b->cfg; }
+            }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    b->cfg = ok ? win : keep;
+    ok = ok && batch_upload_queues(b);
+    hipEventRecord(b->ready, b->stream);
+    for(DNvolume* vol : b->vols) hipStreamWaitEvent(vol->gpu->stream, b->ready, 0);
+    if(best) *best = b->cfg;
+    if(bestMs) *bestMs = winMs;
+    return ok;
 }

@@ -106,237 +106,17 @@ __global__ __launch_bounds__(256) void k_draw(DNdrawParams P)
     }
 }
 
+
+}  // namespace
+
+#include "dn_light.hpp"
+
+namespace {
+
 // ---------------------------------------------------------------------------
-// lighting pass (assets/shaders/voxelLighting.comp:65-288). One lane per stored
-// voxel; a wave takes two consecutive 32-voxel requests (normally the two
-// halves of one chunk's voxel range) from a device work counter.
-
-// One lighting ray: step_map(dir, inv = 1/dir, pos, ignoreFirst = true, ...) as
-// voxelLighting.comp calls it. The nested chunk-in-map loops are measured
-// faster here than the single flattened loop (DN_LIGHT_FLAT); both give the
-// same results bit for bit.
-__device__ __forceinline__ bool light_trace(const DNsceneView& s, RayCtx& c, f3 dir, f3 inv, f3& p, DVoxel& vox,
-                                            f3& ca, float& cm)
-{
-    f3 tn = splat(0.0f);
-#ifdef DN_LIGHT_FLAT
-    return step_map_flat<false>(s, c, dir, inv, p, true, vox, ca, cm, tn);
-#else
-    return step_map<false>(s, c, dir, inv, p, true, tn, vox, ca, cm);
-#endif
-}
-
-struct LCtx {
-    const DNlightParams* P;
-    RayCtx* c;
-    bool firstSample;
-    f3 sunDir, sunStrength;
-};
-
-__device__ void shadow_ray(LCtx& L, f3 p, uint32_t sample, f3& color)
-{
-    const DNlightParams& P = *L.P;
-    DVoxel hv = {};
-    f3 ca;
-    float cm;
-    if(P.numDirectRays == 0)
-    {
-        const float* r = L.firstSample ? P.firstShadow : P.shadowRays[sample];
-        f3 pp = p;
-        if(!light_trace(P.scene, *L.c, ld3(r), ld3(r + 3), pp, hv, ca, cm))
-            color = color + (L.sunStrength * cm + ca);
-        return;
-    }
-    f3 acc = splat(0.0f);
-    for(uint32_t k = 0; k < P.numDirectRays; k++)
-    {
-        f3 pp = p;
-        if(!light_trace(P.scene, *L.c, ld3(P.directRays[k]), ld3(P.directRays[k] + 3), pp, hv, ca, cm))
-            acc = acc + (L.sunStrength * cm + ca);
-    }
-    color = color + acc / (float)P.numDirectRays;
-}
-
-__device__ void specular_ray(LCtx& L, uint32_t thisMapIndex, f3 p, f3 dir, f3 albedo, uint32_t reflectType, f3& color)
-{
-    const DNlightParams& P = *L.P;
-    const DNsceneView& s = P.scene;
-    DVoxel hv = {};
-    f3 lastPos = p;
-    f3 mult = albedo;
-    for(uint32_t i = 0; i < P.specBounceLimit; i++)
-    {
-        f3 ca; float cm;
-        if(light_trace(s, *L.c, dir, vinv(dir), p, hv, ca, cm))
-        {
-            int32_t hx = (int32_t)p.x, hy = (int32_t)p.y, hz = (int32_t)p.z;
-            uint32_t hitIdx = (uint32_t)hx + s.mapX * ((uint32_t)hy + s.mapY * (uint32_t)hz);
-            bool inMap = hx >= 0 && hy >= 0 && hz >= 0 && (uint32_t)hx < s.mapX && (uint32_t)hy < s.mapY &&
-                         (uint32_t)hz < s.mapZ;
-            if(inMap && (s.flags[thisMapIndex] & 4u))
-            {
-                uint32_t bit = 1u << (hitIdx & 31u);
-                if(!(P.visProp[hitIdx >> 5] & bit)) atomicOr(&P.visProp[hitIdx >> 5], bit);
-            }
-            f3 dist = vabs(vfloor(p * 8.0f) - vfloor(lastPos * 8.0f));
-            if(dot3(dist, dist) <= 1.0f) return;
-            const DNmat& hm = s.mats[hv.material];
-            hv.diffuse = hv.diffuse * (1.0f - hm.specular);
-            if(hm.emissive)
-            {
-                color = color + ((hv.albedo * emission_of(hm)) * cm + ca) * mult * albedo;
-                return;
-            }
-            f3 hitColor = hv.diffuse * hv.albedo;
-            color = color + (hitColor * cm + ca) * mult;
-            if(hm.specular == 0.0f) return;
-            mult = mult * ((hv.albedo * cm) * hm.specular);
-            reflectType = hm.reflectType;
-            lastPos = p;
-            dir = vreflect(dir, hv.normal);
-        }
-        else if(dot3(dir, L.sunDir) > 0.99f)
-        {
-            color = color + (L.sunStrength * cm + ca);
-            return;
-        }
-        else
-        {
-            f3 src = reflectType == 1u ? sky_color(s, dir) : L.sunStrength;
-            color = color + (src * cm + ca) * mult;
-            return;
-        }
-    }
-}
-
-__device__ void diffuse_ray(LCtx& L, f3 normal, f3 p, const DVoxel& initial, uint32_t sample, f3& color)
-{
-    const DNlightParams& P = *L.P;
-    const DNsceneView& s = P.scene;
-    DVoxel hv = initial;
-    f3 hitNormal = normal;
-    float hitSpec = 0.0f;
-    uint32_t hitShine = 0;
-    f3 newColor = splat(1.0f);
-    f3 lastPos = p;
-    f3 lastDir = splat(0.0f);
-    for(uint32_t i = 0; i < P.diffuseBounceLimit; i++)
-    {
-        f3 dir;
-        if(i > 0 && (P.rnd.glossyPick[sample][i] + 1.0f) * 0.5f < hitSpec)
-            dir = vnorm(vreflect(lastDir, hitNormal) * (float)hitShine + ld3(P.rnd.glossyDir[i]));
-        else if(L.firstSample)
-            dir = vnorm(hitNormal) + DN_EPS;
-        else
-            dir = vnorm(hitNormal + ld3(P.rnd.diffuseDir[sample][i])) + DN_EPS;
-        f3 ca;
-        float cm;
-        DVoxel nv = {};
-        bool hit = light_trace(s, *L.c, dir, vinv(dir), p, nv, ca, cm);
-        if(hit)
-        {
-            hv = nv;
-            hitNormal = hv.normal;
-            const DNmat& hm = s.mats[hv.material];
-            hitSpec = hm.specular;
-            hitShine = hm.shininess;
-            f3 dist = vabs(vfloor(lastPos * 8.0f) - vfloor(p * 8.0f));
-            if(dot3(dist, dist) < 1.0f) return;
-            if(hm.emissive)
-            {
-                color = color + newColor * ((hv.albedo * emission_of(hm)) * cm + ca);
-                return;
-            }
-            newColor = newColor * (hv.albedo * cm + ca);
-        }
-        else
-        {
-            float cosv = gmax(dot3(dir, L.sunDir), 0.0f);
-            color = color + (((newColor * cosv) * L.sunStrength) * cm + ca);
-            return;
-        }
-        lastDir = dir;
-    }
-}
-
-__device__ __forceinline__ uint32_t rne_u(float x) { return (uint32_t)rintf(x); }
-
-__device__ void light_voxel(const DNlightParams& P, uint32_t r, uint32_t lane32, RayCtx& c, uint32_t& lit)
-{
-    const DNsceneView& s = P.scene;
-    const uint32_t req = P.requests[r];
-    const uint32_t mapIndex = req >> 4, block = req & 15u;
-    const uint32_t voxNum = lane32 + block * 32u;
-    const DNchunkRec* rec = s.chunks + mapIndex;
-    const size_t slot = (size_t)r * 32u + lane32;
-    int32_t sel = select_in(rec->mask, voxNum);
-    const bool dbg = slot < P.debugSlots;
-    if(sel < 0)
-    {
-        if(dbg && P.debugIndex) P.debugIndex[slot] = 0xFFFFFFFFu;
-        return;
-    }
-    if(voxNum == 0)
-        atomicOr(&P.litBits[mapIndex >> 5], 1u << (mapIndex & 31u));
-    c.lastVoxID = 255u; c.lastVoxRefract = 1.0f;
-    const int32_t cx = sel % 8, cy = (sel / 8) % 8, cz = sel / 64;
-    const uint32_t vi = rec->voxelIndex + voxNum;
-    const uint4 cw = s.voxels[vi];
-    const DVoxel me = decompress(cw);
-    const DNmat& mat = s.mats[me.material];
-    const uint32_t ns = rec->samples;
-    const float indirect = (float)(ns < P.maxDiffuseSamples ? ns : P.maxDiffuseSamples);
-    LCtx L;
-    L.P = &P; L.c = &c;
-    L.firstSample = indirect == 0.0f;
-    L.sunDir = ld3(P.sunDir);
-    L.sunStrength = ld3(s.sunStrength);
-    f3 p = (mk((float)cx, (float)cy, (float)cz) * 0.125f + mk((float)rec->pos[0], (float)rec->pos[1], (float)rec->pos[2])) +
-           splat(0.0625f);
-    p = p + splat(0.0625f - DN_EPS) * me.normal;
-    f3 spec = splat(0.0f), diff = splat(0.0f);
-    f3 view = p - ld3(P.camPos);
-    if(mat.specular > 0.0f && dot3(view, me.normal) < 0.0f && mat.reflectType <= 1u)
-    {
-        f3 refl = vreflect(vnorm(view), me.normal);
-        for(uint32_t i = 0; i < P.numSpecRays; i++)
-        {
-            f3 sd = vnorm(refl * (float)mat.shininess + ld3(P.specPoints + 3 * i)) + DN_EPS;
-            specular_ray(L, mapIndex, p, sd, me.albedo, mat.reflectType, spec);
-        }
-        spec = spec / (float)P.numSpecRays;
-    }
-    if(mat.specular < 1.0f)
-    {
-        for(uint32_t i = 0; i < P.numDiffuseSamples; i++)
-        {
-            diff = diff + ld3(P.ambient);
-            diffuse_ray(L, me.normal + DN_EPS, p, me, i, diff);
-            shadow_ray(L, p, i, diff);
-        }
-        diff = (me.diffuse * indirect + diff) / (indirect + (float)P.numDiffuseSamples);
-    }
-    if(dbg && P.debugValues)
-    {
-        float* dv = P.debugValues + slot * 6;
-        dv[0] = diff.x; dv[1] = diff.y; dv[2] = diff.z; dv[3] = spec.x; dv[4] = spec.y; dv[5] = spec.z;
-    }
-    if(dbg && P.debugIndex) P.debugIndex[slot] = vi;
-    spec = vclamp(spec, 0.0f, 1.0f);
-    diff = vclamp(diff, 0.0f, 1.0f);
-    const uint32_t wdx = rne_u(diff.x * 65535.0f), wdy = rne_u(diff.y * 65535.0f), wdz = rne_u(diff.z * 65535.0f);
-    const uint32_t ar = rne_u(me.albedo.x * 255.0f) & 255u, ag = rne_u(me.albedo.y * 255.0f) & 255u,
-                   ab = rne_u(me.albedo.z * 255.0f) & 255u;
-    const uint32_t sx = rne_u(spec.x * 255.0f) & 255u, sy = rne_u(spec.y * 255.0f) & 255u,
-                   sz = rne_u(spec.z * 255.0f) & 255u;
-    uint4 o;
-    o.x = cw.x;
-    o.y = (ar << 24) | (ag << 16) | (ab << 8) | sx;
-    o.z = (sy << 24) | (sz << 16) | (wdx & 0xFFFFu);
-    o.w = ((wdy & 0xFFFFu) << 16) | (wdz & 0xFFFFu);
-    P.voxelsOut[vi] = o;
-    lit++;
-}
+// lighting pass: one lane per stored voxel; a wave takes two consecutive
+// 32-voxel requests (normally the two halves of one chunk's voxel range) from
+// a device work counter.
 
 __global__ __launch_bounds__(256) void k_lighting(DNlightParams P)
 {
@@ -352,7 +132,7 @@ __global__ __launch_bounds__(256) void k_lighting(DNlightParams P)
         base = __shfl(base, 0, 64);
         if(base >= nreq) break;
         const uint32_t r = base + (lane >> 5);
-        if(r < nreq) light_voxel(P, r, lane & 31u, c, lit);
+        if(r < nreq) light_voxel(P, r, lane & 31u, c, lit, false);
     }
     if(P.counters)
     {
@@ -364,15 +144,12 @@ __global__ __launch_bounds__(256) void k_lighting(DNlightParams P)
     }
 }
 
-// Pass epilogue over the tile bitmaps: clear the visible bit and bump the
-// sample count of lit chunks, then set the visible bit of chunks reached by
-// specular rays from visible chunks; reset the work counter.
-__global__ __launch_bounds__(256) void k_light_epilogue(uint32_t* flags, DNchunkRec* chunks, uint32_t* litBits,
-                                                        uint32_t* visProp, uint32_t words, uint32_t* workCounter)
+// Pass epilogue over one 32-tile word of the bitmaps: clear the visible bit and
+// bump the sample count of lit chunks, then set the visible bit of chunks
+// reached by specular rays from visible chunks.
+__device__ __forceinline__ void epilogue_word(uint32_t* flags, DNchunkRec* chunks, uint32_t* litBits,
+                                              uint32_t* visProp, uint32_t w)
 {
-    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    if(w == 0) *workCounter = 0u;
-    if(w >= words) return;
     uint32_t lit = litBits[w], prop = visProp[w];
     if(!(lit | prop)) return;
     uint32_t any = lit | prop;
@@ -390,19 +167,35 @@ __global__ __launch_bounds__(256) void k_light_epilogue(uint32_t* flags, DNchunk
     visProp[w] = 0u;
 }
 
-// Copies the lit voxels of this pass from the output pool back into the pool
-// (only used when the pass wrote to a separate buffer).
-__global__ __launch_bounds__(256) void k_copy_back(const uint32_t* requests, const uint32_t* numRequests,
-                                                   const DNchunkRec* chunks, const uint4* src, uint4* dst)
+// The epilogue of one pass; also resets the pass's work counter.
+__global__ __launch_bounds__(256) void k_light_epilogue(uint32_t* flags, DNchunkRec* chunks, uint32_t* litBits,
+                                                        uint32_t* visProp, uint32_t words, uint32_t* workCounter)
 {
-    const uint32_t n = *numRequests;
-    for(uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < n * 32u; g += gridDim.x * blockDim.x)
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if(w == 0) *workCounter = 0u;
+    if(w < words) epilogue_word(flags, chunks, litBits, visProp, w);
+}
+
+// Copies the lit voxels of a pass from the output pool back into the pool
+// (only used when the pass wrote to a separate buffer); grid-stride over the
+// n * 32 voxel slots of the requests.
+__device__ __forceinline__ void copy_back_range(const uint32_t* requests, uint32_t n, const DNchunkRec* chunks,
+                                                const uint4* src, uint4* dst, uint32_t first, uint32_t stride)
+{
+    for(uint32_t g = first; g < n * 32u; g += stride)
     {
         uint32_t req = requests[g >> 5];
         uint32_t voxNum = (g & 31u) + (req & 15u) * 32u;
         const DNchunkRec* rec = chunks + (req >> 4);
         if(voxNum < rec->numVoxels) dst[rec->voxelIndex + voxNum] = src[rec->voxelIndex + voxNum];
     }
+}
+
+__global__ __launch_bounds__(256) void k_copy_back(const uint32_t* requests, const uint32_t* numRequests,
+                                                   const DNchunkRec* chunks, const uint4* src, uint4* dst)
+{
+    copy_back_range(requests, *numRequests, chunks, src, dst, blockIdx.x * blockDim.x + threadIdx.x,
+                    gridDim.x * blockDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -560,7 +353,146 @@ __global__ __launch_bounds__(256) void k_mark_visible(uint32_t* flags, const uin
     }
 }
 
+// ---------------------------------------------------------------------------
+// Batched lighting over many volumes in one persistent launch. Results do not
+// depend on the work split: every voxel is lit by the same per-voxel code from
+// the same inputs as in the single-volume pass.
+
+// Per queue: units of its volumes (ceil(numRequests / unit)) and their
+// exclusive prefix over the queue's volume list; resets the queue counters.
+// One block of 1024.
+__global__ __launch_bounds__(1024) void k_batch_prefix(DNbatchParams B)
+{
+    __shared__ uint32_t waveSums[16];
+    __shared__ uint32_t carry;
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    if(t < B.queues) B.queueCounters[t] = 0u;
+    for(uint32_t q = 0; q < B.queues; q++)
+    {
+        const uint32_t len = B.queueLen[q];
+        const uint32_t* ids = B.queueScenes + (size_t)q * B.numScenes;
+        uint32_t* pre = B.queuePrefix + (size_t)q * (B.numScenes + 1u);
+        if(t == 0) carry = 0u;
+        __syncthreads();
+        for(uint32_t base = 0; base < len; base += 1024u)
+        {
+            const uint32_t i = base + t;
+            uint32_t u = 0;
+            if(i < len) u = (*B.scenes[ids[i]].numRequests + B.unit - 1u) / B.unit;
+            const uint32_t incl = wave_incl_scan(u);
+            if(lane == 63u) waveSums[wave] = incl;
+            __syncthreads();
+            uint32_t before = carry;
+            for(uint32_t w = 0; w < wave; w++) before += waveSums[w];
+            if(i < len) pre[i] = before + incl - u;
+            __syncthreads();
+            if(t == 1023u) carry = before + incl;
+            __syncthreads();
+        }
+        if(t == 0) pre[len] = carry;
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ void flush_counters(uint64_t* cnt, RayCtx& c, uint32_t& lit)
+{
+    if(cnt)
+    {
+        wave_add_u64(cnt + CNT_RAYS, c.cRays);
+        wave_add_u64(cnt + CNT_MAPSTEPS, c.cMap);
+        wave_add_u64(cnt + CNT_VOXSTEPS, c.cVox);
+        wave_add_u64(cnt + CNT_HITS, c.cHits);
+        wave_add_u64(cnt + CNT_LIT, lit);
+    }
+    c.cRays = c.cMap = c.cVox = c.cHits = 0;
+    lit = 0;
+}
+
+// Persistent: each wave takes units (`unit` requests of one volume, two
+// requests per 64-lane pass) from its home queue, then from the others.
+// Workgroups are dealt to the 8 XCDs round-robin by workgroup id, so with 8
+// queues the home queue of workgroup b is b % 8 and a volume's units stay on
+// one XCD (one L2) while its queue lasts.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_lighting_batch(DNbatchParams B)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t home = blockIdx.x % B.queues;
+    const bool dry = B.dry != 0u;
+    RayCtx c;
+    ctx_init(c);
+    uint32_t lit = 0;
+    uint32_t cur = 0xFFFFFFFFu;
+    for(uint32_t k = 0; k < B.queues; k++)
+    {
+        const uint32_t q = (home + k) % B.queues;
+        const uint32_t len = B.queueLen[q];
+        if(len == 0) continue;
+        const uint32_t* pre = B.queuePrefix + (size_t)q * (B.numScenes + 1u);
+        const uint32_t total = pre[len];
+        for(;;)
+        {
+            uint32_t u = 0;
+            if(lane == 0) u = atomicAdd(&B.queueCounters[q], 1u);
+            u = __builtin_amdgcn_readfirstlane(__shfl(u, 0, 64));
+            if(u >= total) break;
+            uint32_t lo = 0, hi = len - 1u;           // last volume with pre[i] <= u
+            while(lo < hi)
+            {
+                const uint32_t mid = (lo + hi + 1u) >> 1;
+                if(pre[mid] <= u) lo = mid;
+                else hi = mid - 1u;
+            }
+            const uint32_t sid = __builtin_amdgcn_readfirstlane(B.queueScenes[(size_t)q * B.numScenes + lo]);
+            if(sid != cur)
+            {
+                if(cur != 0xFFFFFFFFu && !dry) flush_counters(B.scenes[cur].counters, c, lit);
+                cur = sid;
+            }
+            const DNlightParams& P = B.scenes[sid];
+            const uint32_t nreq = *P.numRequests;
+            const uint32_t r0 = (u - pre[lo]) * B.unit;
+            for(uint32_t j = 0; j < B.unit; j += 2u)
+            {
+                const uint32_t r = r0 + j + (lane >> 5);
+                if(r < nreq) light_voxel(P, r, lane & 31u, c, lit, dry);
+            }
+        }
+    }
+    if(cur != 0xFFFFFFFFu && !dry) flush_counters(B.scenes[cur].counters, c, lit);
+}
+
+// Copy-back and epilogue of every volume of a batch: blockIdx.y = volume.
+__global__ __launch_bounds__(256) void k_batch_copy_back(const DNlightParams* scenes)
+{
+    const DNlightParams& P = scenes[blockIdx.y];
+    if(!P.copyBack) return;
+    copy_back_range(P.requests, *P.numRequests, P.scene.chunks, P.voxelsOut, (uint4*)P.scene.voxels,
+                    blockIdx.x * 256u + threadIdx.x, gridDim.x * 256u);
+}
+
+__global__ __launch_bounds__(256) void k_batch_epilogue(const DNlightParams* scenes)
+{
+    const DNlightParams& P = scenes[blockIdx.y];
+    const uint32_t words = (P.tiles + 31u) / 32u;
+    for(uint32_t w = blockIdx.x * 256u + threadIdx.x; w < words; w += gridDim.x * 256u)
+        epilogue_word(P.scene.flags, (DNchunkRec*)P.scene.chunks, P.litBits, P.visProp, w);
+}
+
 int g_lightingBlocks = 0;
+int g_cus = 0;
+
+int cu_count()
+{
+    if(g_cus == 0)
+    {
+        int dev = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&g_cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if(g_cus < 1) g_cus = 1;
+    }
+    return g_cus;
+}
 
 }  // namespace
 
@@ -603,6 +535,39 @@ int dn_launch_lighting_epilogue(const DNlightParams& p, uint32_t tiles, const ui
     uint32_t words = (tiles + 31) / 32;
     hipLaunchKernelGGL(k_light_epilogue, dim3((words + 255) / 256), dim3(256), 0, stream, p.scene.flags,
                        (DNchunkRec*)p.scene.chunks, p.litBits, p.visProp, words, p.workCounter);
+    return (int)hipGetLastError();
+}
+
+int dn_launch_batch(const DNbatchParams& b, uint32_t blockThreads, uint32_t blocksPerCU, uint32_t maxTiles,
+                    uint32_t maxRequests, bool epilogue, hipStream_t stream)
+{
+    if(b.numScenes == 0) return 0;
+    if(b.numScenes > 65535u || b.unit < 2u || (b.unit & 1u) || b.queues < 1u || b.queues > DN_BATCH_MAX_QUEUES ||
+       (blockThreads != 64u && blockThreads != 256u))
+        return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_batch_prefix, dim3(1), dim3(1024), 0, stream, b);
+    static int perCU64 = 0, perCU256 = 0;
+    int& perCU = blockThreads == 64u ? perCU64 : perCU256;
+    if(perCU == 0)
+    {
+        if(blockThreads == 64u) hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_lighting_batch<64>, 64, 0);
+        else hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_lighting_batch<256>, 256, 0);
+        if(perCU < 1) perCU = 1;
+    }
+    uint32_t per = (uint32_t)perCU;
+    if(blocksPerCU && blocksPerCU < per) per = blocksPerCU;
+    const uint32_t blocks = (uint32_t)cu_count() * per;
+    if(blockThreads == 64u) hipLaunchKernelGGL(k_lighting_batch<64>, dim3(blocks), dim3(64), 0, stream, b);
+    else hipLaunchKernelGGL(k_lighting_batch<256>, dim3(blocks), dim3(256), 0, stream, b);
+    if(epilogue && !b.dry)
+    {
+        uint32_t cx = (maxRequests * 32u + 255u) / 256u;
+        cx = cx < 1u ? 1u : (cx > 64u ? 64u : cx);
+        hipLaunchKernelGGL(k_batch_copy_back, dim3(cx, b.numScenes), dim3(256), 0, stream, b.scenes);
+        uint32_t ex = ((maxTiles + 31u) / 32u + 255u) / 256u;
+        ex = ex < 1u ? 1u : (ex > 64u ? 64u : ex);
+        hipLaunchKernelGGL(k_batch_epilogue, dim3(ex, b.numScenes), dim3(256), 0, stream, b.scenes);
+    }
     return (int)hipGetLastError();
 }
 

@@ -95,6 +95,7 @@ class Scene:
     width: int = 256
     height: int = 256
     params: dict = field(default_factory=dict)
+    visible: int = 0                     # chunks marked visible per frame (batched workload)
 
     @property
     def occupancy(self) -> float:
@@ -225,3 +226,48 @@ def config2() -> Scene:
     params = {"numSpecularRays": 32, "numDirectRays": 32, "diffuseBounceLimit": 4}
     return Scene("config2", (64, 32, 64), nw, aw, mats, cam_pos=(4.0, 12.0, 4.0), cam_orient=(30.0, 45.0, 0.0),
                  width=1920, height=1080, params=params)
+
+
+def batch_scene(index: int, max_grid: int = 48) -> Scene:
+    """Scene `index` of the batched uneven workload (seed = index): chunk grid
+    uniform in [4, max_grid] per axis, fBm heightfield at an occupancy uniform in
+    [5%, 60%], 2-16 materials (the last one emissive, on ~0.5% of columns), and
+    a visible-chunk budget log-uniform in [10, 40000] (capped by the grid later
+    by the exposed-chunk count). Voxels of a column share the column's normal
+    and material, so the volume is built from 2D arrays."""
+    def u(k):
+        return float(uniform01(index, k, seed=977))
+    span = max_grid - 4 + 1
+    gx, gy, gz = (4 + min(span - 1, int(span * u(k))) for k in range(3))
+    occ = 0.05 + 0.55 * u(3)
+    n_mat = 2 + min(14, int(15 * u(4)))
+    visible = int(round(np.exp(np.log(10.0) + (np.log(40000.0) - np.log(10.0)) * u(5))))
+    X, Y, Z = gx * 8, gy * 8, gz * 8
+    gxs, gzs = np.meshgrid(np.arange(X, dtype=np.float64), np.arange(Z, dtype=np.float64))
+    f = fbm(gxs / 48.0, gzs / 48.0, seed=index * 7 + 1)
+    amp = 0.3 * Y
+    lo, hi = -amp, float(Y)
+    for _ in range(50):
+        h0 = 0.5 * (lo + hi)
+        if np.clip(h0 + amp * f, 0, Y).mean() / Y < occ:
+            lo = h0
+        else:
+            hi = h0
+    h = h0 + amp * f
+    hz, hx = np.gradient(h)
+    nrm = np.stack([-hx, np.ones_like(h), -hz], axis=-1)
+    nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
+    ix, iz = np.meshgrid(np.arange(X), np.arange(Z))
+    mat = hash_u32(ix, iz, seed=index + 5) % np.uint32(max(1, n_mat - 1))
+    emissive = hash_u32(ix, iz, seed=index + 6) < np.uint32(0.005 * 4294967296.0)
+    mat = np.where(emissive, n_mat - 1, mat).astype(np.uint32)
+    mats = _materials(n_mat, index + 11, n_mat - 1, 1.0)
+    col_nw = pack_normal_word(mat, nrm)                              # (Z, X)
+    alb = np.array([m.albedo for m in mats], dtype=np.uint32)
+    col_aw = (alb[mat, 0] << 24) | (alb[mat, 1] << 16) | (alb[mat, 2] << 8)
+    solid = np.arange(Y)[None, :, None] < h[:, None, :]              # (Z, Y, X)
+    nw = np.where(solid, col_nw[:, None, :], np.uint32(0xFFFFFFFF)).astype(np.uint32)
+    aw = np.where(solid, col_aw[:, None, :], np.uint32(0)).astype(np.uint32)
+    params = {"numSpecularRays": 16, "numDirectRays": 16, "diffuseBounceLimit": 4}
+    return Scene(f"batch{index}", (gx, gy, gz), nw, aw, mats, cam_pos=(gx * 0.5, gy * 0.9, gz * 0.5),
+                 cam_orient=(45.0, 45.0, 0.0), params=params, visible=visible)

@@ -182,6 +182,147 @@ class Volume:
         return out[:n].copy()
 
 
+class Batch:
+    """DNXbatch: the lighting pass of many volumes in one persistent, graph-replayed launch."""
+
+    def __init__(self, volumes):
+        self.lib = _lib.load()
+        self.volumes = list(volumes)
+        arr = (ctypes.POINTER(_lib.DNvolume) * len(self.volumes))(*[v.ptr for v in self.volumes])
+        self.ptr = self.lib.DNX_batch_create(arr, len(self.volumes))
+        if not self.ptr:
+            raise RuntimeError("DNX_batch_create failed")
+
+    def close(self) -> None:
+        if getattr(self, "ptr", None):
+            self.lib.DNX_batch_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def config(self) -> dict:
+        c = _lib.DNXbatchConfig()
+        self.lib.DNX_batch_get_config(self.ptr, ctypes.byref(c))
+        return {f: getattr(c, f) for f, _ in c._fields_}
+
+    def set_config(self, requests_per_unit=8, block_threads=256, blocks_per_cu=0, xcd_queues=1) -> None:
+        c = _lib.DNXbatchConfig(requests_per_unit, block_threads, blocks_per_cu, xcd_queues)
+        if not self.lib.DNX_batch_set_config(self.ptr, ctypes.byref(c)):
+            raise ValueError("invalid batch configuration")
+
+    def update_lighting(self, num_diffuse: int = 1, max_diffuse: int = 1000, time: float = 123.456) -> None:
+        if not self.lib.DNX_batch_update_lighting(self.ptr, num_diffuse, max_diffuse, time):
+            raise RuntimeError("DNX_batch_update_lighting failed")
+
+    def autotune(self, num_diffuse: int = 1, max_diffuse: int = 1000, time: float = 123.456, repeats: int = 3):
+        c, ms = _lib.DNXbatchConfig(), ctypes.c_float()
+        if not self.lib.DNX_batch_autotune(self.ptr, num_diffuse, max_diffuse, time, repeats, ctypes.byref(c),
+                                           ctypes.byref(ms)):
+            raise RuntimeError("DNX_batch_autotune failed")
+        return {f: getattr(c, f) for f, _ in c._fields_}, ms.value
+
+    def timing(self, reset: bool = False):
+        n = ctypes.c_uint32()
+        ms = self.lib.DNX_batch_timing(self.ptr, ctypes.byref(n), reset)
+        return ms, n.value
+
+
+RC_NPARAM = 12288
+RC_OFFSETS = ((0, 64, 48, 38), (3072, 64, 64, 64), (7168, 64, 64, 64), (11264, 16, 64, 3))  # off, rows, ld, used
+
+
+def rc_pack_weights(mats) -> np.ndarray:
+    """[W0 (64x38), W1, W2 (64x64), W3 (3x64)] -> the padded 12288-float layout."""
+    out = np.zeros(RC_NPARAM, np.float32)
+    for (off, rows, ld, _), m in zip(RC_OFFSETS, mats):
+        blk = np.zeros((rows, ld), np.float32)
+        blk[:m.shape[0], :m.shape[1]] = m
+        out[off:off + rows * ld] = blk.reshape(-1)
+    return out
+
+
+def rc_unpack_weights(w: np.ndarray):
+    mats = []
+    for off, rows, ld, used in RC_OFFSETS:
+        blk = np.asarray(w[off:off + rows * ld]).reshape(rows, ld)
+        mats.append(blk[:3, :] if used == 3 else blk[:, :used])
+    return mats
+
+
+class RadianceCache:
+    """DNXradianceCache of one volume."""
+
+    def __init__(self, vol: Volume, **cfg):
+        self.lib = _lib.load()
+        self.vol = vol
+        c = _lib.DNXrcConfig()
+        self.lib.DNX_rc_default_config(ctypes.byref(c))
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        self.ptr = self.lib.DNX_rc_create(vol.ptr, ctypes.byref(c))
+        if not self.ptr:
+            raise RuntimeError("DNX_rc_create failed")
+        self.config = {f: getattr(c, f) for f, _ in c._fields_}
+        self.table_floats = 16 * 2 * (1 << c.log2Table)
+
+    def close(self) -> None:
+        if getattr(self, "ptr", None):
+            self.lib.DNX_rc_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def train_frame(self, time: float, steps: int = 1) -> None:
+        if not self.lib.DNX_rc_train_frame(self.ptr, time, steps):
+            raise RuntimeError("DNX_rc_train_frame failed")
+
+    def predict_requests(self, out: DeviceBuffer, request_capacity: int) -> None:
+        if not self.lib.DNX_rc_predict_requests(self.ptr, out.ptr, request_capacity):
+            raise RuntimeError("DNX_rc_predict_requests failed")
+
+    def train_samples(self, samples: DeviceBuffer, n: int, step: bool) -> None:
+        if not self.lib.DNX_rc_train_samples(self.ptr, samples.ptr, n, step):
+            raise RuntimeError("DNX_rc_train_samples failed")
+
+    def predict_samples(self, samples: DeviceBuffer, n: int, out: DeviceBuffer) -> None:
+        if not self.lib.DNX_rc_predict_samples(self.ptr, samples.ptr, n, out.ptr):
+            raise RuntimeError("DNX_rc_predict_samples failed")
+
+    def set_params(self, weights: np.ndarray, table: np.ndarray) -> None:
+        w = np.ascontiguousarray(weights, np.float32)
+        t = np.ascontiguousarray(table, np.float32)
+        assert w.size == RC_NPARAM and t.size == self.table_floats
+        if not self.lib.DNX_rc_set_params(self.ptr, w.ctypes.data, t.ctypes.data):
+            raise RuntimeError("DNX_rc_set_params failed")
+
+    def get_params(self):
+        """(weights, table, weight gradients, table gradients) as float32 arrays."""
+        w, g = np.zeros(RC_NPARAM, np.float32), np.zeros(RC_NPARAM, np.float32)
+        t, tg = np.zeros(self.table_floats, np.float32), np.zeros(self.table_floats, np.float32)
+        if not self.lib.DNX_rc_get_params(self.ptr, w.ctypes.data, t.ctypes.data, g.ctypes.data, tg.ctypes.data):
+            raise RuntimeError("DNX_rc_get_params failed")
+        return w, t, g, tg
+
+    def last_samples(self) -> np.ndarray:
+        n = ctypes.c_uint32()
+        p = self.lib.DNX_rc_last_samples(self.ptr, ctypes.byref(n))
+        out = np.zeros((n.value, 16), np.float32)
+        if n.value:
+            from ._hip import hip, check
+            self.vol.synchronize()
+            check(hip().hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(p), out.nbytes, 2), "samples")
+        return out
+
+
 def draw_matrices(view: DNmat4, proj: DNmat4):
     """(invView, invCenteredView * invProjection) as the draw kernel receives them."""
     lib = _lib.load()

@@ -322,9 +322,87 @@ void DNX_set_draw_hit_output(DNvolume* vol, DNXpixelHit* out, size_t capacityPix
  * spec rgb) and of pool indices (0xFFFFFFFF for empty lanes). */
 void DNX_set_lighting_debug_output(DNvolume* vol, float* values, uint32_t* voxelIndices, size_t requestCapacity);
 
+/* This pass's diffuse estimate of each voxel before it enters the running mean
+ * (voxelLighting.comp:254-261 summed over the samples, divided by their count):
+ * device array of 32 * requestCapacity float4 (rgb, 1 for a stored voxel and 0
+ * for an empty lane). NULL disables. */
+void DNX_set_lighting_sample_output(DNvolume* vol, float* samples, size_t requestCapacity);
+
 /* Camera matrices for the draw kernel, from DN_draw's arguments: the inverse
  * view matrix and invCenteredView * invProjection (voxel.c:844-853). */
 void DNX_draw_matrices(DNmat4 view, DNmat4 projection, DNmat4* invView, DNmat4* invViewProj);
+
+/* Batched lighting: one persistent launch lights every volume of a batch, with
+ * results bit-identical to calling DN_update_lighting on each volume in turn.
+ * No counterpart in the reference (whose lighting pass is one glDispatchCompute
+ * per volume, voxel.c:883-952). Each volume must have been DN_sync_gpu'd; the
+ * batch orders itself after every volume's stream and each volume's stream
+ * after the batch. The frame's launches are captured once into a hipGraph per
+ * configuration and replayed. */
+typedef struct DNXbatch DNXbatch;
+typedef struct DNXbatchConfig {
+    uint32_t requestsPerUnit;   /* requests one wave takes per queue grab: even, 2..64 */
+    uint32_t blockThreads;      /* 64 or 256 */
+    uint32_t blocksPerCU;       /* 0: as many as fit */
+    uint32_t xcdQueues;         /* 1: one queue per XCD (8), volumes dealt round-robin; 0: one queue */
+} DNXbatchConfig;
+DNXbatch* DNX_batch_create(DNvolume* const* volumes, uint32_t count);
+void DNX_batch_destroy(DNXbatch* batch);
+bool DNX_batch_set_config(DNXbatch* batch, const DNXbatchConfig* config);
+void DNX_batch_get_config(DNXbatch* batch, DNXbatchConfig* config);
+/* Same arguments and semantics as DN_update_lighting, for every volume. */
+bool DNX_batch_update_lighting(DNXbatch* batch, int numDiffuseSamples, int maxDiffuseSamples, float time);
+/* Times every candidate configuration with compute-only passes (no stores) of
+ * the current frame's requests, `repeats` launches each, keeps the fastest and
+ * returns it. Volume state is left as it was. */
+bool DNX_batch_autotune(DNXbatch* batch, int numDiffuseSamples, int maxDiffuseSamples, float time, int repeats,
+                        DNXbatchConfig* best, float* bestMs);
+/* Milliseconds of the batched launches (HIP events around each graph launch)
+ * since the last reset, and their count. */
+double DNX_batch_timing(DNXbatch* batch, uint32_t* launches, bool reset);
+
+/* Learned radiance cache (no counterpart in the reference): a multiresolution
+ * hash encoding of the voxel position (16 levels x 2 features, 2^log2Table
+ * entries per level) plus the voxel normal and albedo feed an MLP of four
+ * weight matrices (3 hidden layers of 64, ReLU, linear RGB output, no biases),
+ * bf16 operands with fp32 accumulation on the matrix cores. It is trained
+ * online with Adam on path-traced per-voxel diffuse samples drawn with
+ * per-sample random numbers, and predicts each voxel's diffuse irradiance
+ * (the quantity DN_update_lighting accumulates). The plain accumulator keeps
+ * running beside it. A sample row is 16 floats: position in voxel units (3),
+ * normal (3), albedo (3), target irradiance (3), valid 0/1, 3 unused. */
+typedef struct DNXradianceCache DNXradianceCache;
+typedef struct DNXrcConfig {
+    uint32_t log2Table;        /* 19 */
+    uint32_t nMin, nMax;       /* coarsest and finest grid resolution: 16, 256 */
+    float extent;              /* positions / extent fall in [0,1); 0: the longest map side in voxels */
+    float lr, beta1, beta2, eps;   /* Adam: 1e-2, 0.9, 0.99, 1e-10 */
+    uint32_t samplesPerFrame;  /* 65536 (multiple of 16) */
+    uint32_t seed;             /* weight initialisation and sample draws */
+} DNXrcConfig;
+void DNX_rc_default_config(DNXrcConfig* config);
+DNXradianceCache* DNX_rc_create(DNvolume* vol, const DNXrcConfig* config /* NULL: defaults */);
+void DNX_rc_destroy(DNXradianceCache* rc);
+/* After DN_sync_gpu: path-traces samplesPerFrame samples on voxels drawn from
+ * the volume's current lighting requests, then takes `steps` Adam steps on them. */
+bool DNX_rc_train_frame(DNXradianceCache* rc, float time, int steps);
+/* Predicted irradiance for every slot (32 per request) of the current request
+ * list: device array of 32 * requestCapacity float4 (rgb, 1 for a stored voxel). */
+bool DNX_rc_predict_requests(DNXradianceCache* rc, float* out, size_t requestCapacity);
+/* Explicit samples (device rows, n a multiple of 16): gradients of the mean
+ * squared-error loss accumulate until an Adam step (`step`) or DNX_rc_set_params. */
+bool DNX_rc_train_samples(DNXradianceCache* rc, const float* samples, uint32_t n, bool step);
+bool DNX_rc_predict_samples(DNXradianceCache* rc, const float* samples, uint32_t n, float* out /* device float4 */);
+/* Host copies of the fp32 master parameters (weights: 12288 floats, the padded
+ * row-major W0 64x48, W1 64x64, W2 64x64, W3 16x64; table: 16 x 2^log2Table x 2).
+ * set_params also clears the gradients and the Adam state. */
+bool DNX_rc_set_params(DNXradianceCache* rc, const float* weights, const float* table);
+bool DNX_rc_get_params(DNXradianceCache* rc, float* weights, float* table, float* weightGrad, float* tableGrad);
+/* Device rows of the samples of the last DNX_rc_train_frame, and their count. */
+const float* DNX_rc_last_samples(DNXradianceCache* rc, uint32_t* n);
+/* dst[i] += (double)src[i] on the volume's stream (device arrays): FP64
+ * accumulation of per-frame samples for reference estimates. */
+bool DNX_accumulate_f64(DNvolume* vol, const float* src, double* dst, size_t n);
 
 #ifdef __cplusplus
 }
