@@ -187,14 +187,229 @@ def bench_batch(args) -> int:
     return 0
 
 
+def catmull_rom(points, u: float):
+    """Closed Catmull-Rom spline through `points` at parameter u in [0, 1)."""
+    import numpy as np
+    n = len(points)
+    x = (u % 1.0) * n
+    i = int(x)
+    t = x - i
+    p0, p1, p2, p3 = (np.asarray(points[(i + k) % n], np.float64) for k in (-1, 0, 1, 2))
+    pos = 0.5 * (2 * p1 + (-p0 + p2) * t + (2 * p0 - 5 * p1 + 4 * p2 - p3) * t * t + (-p0 + 3 * p1 - 3 * p2 + p3) * t ** 3)
+    tan = 0.5 * ((-p0 + p2) + 2 * (2 * p0 - 5 * p1 + 4 * p2 - p3) * t + 3 * (-p0 + 3 * p1 - 3 * p2 + p3) * t * t)
+    return pos, tan
+
+
+def bench_edits(args) -> int:
+    """Dynamic edit streaming on the 16^3-chunk fBm map: each frame 256 seeded
+    sphere carve/fill edits (radius U[3,24], centre uniform over the map,
+    50/50, seed 4 + frame), then the reference's frame order: DN_draw (256x256,
+    camera on a seeded closed Catmull-Rom spline), DN_sync_gpu (re-uploads only
+    the dirty chunks, whose lighting history restarts), DN_update_lighting.
+    Frame time is host wall clock from the first edit to the end of the
+    lighting pass."""
+    import numpy as np
+
+    from doonengine_amd import scenes
+    from doonengine_amd._hip import DeviceBuffer
+    from doonengine_amd.scenes import uniform01
+    from doonengine_amd.volume import KIND_LIGHTING, Volume
+
+    sc = scenes.config1()
+    vol = Volume(sc.map_size, min_chunks=4096)
+    sc.apply(vol)
+    vol.request_all()
+    vol.sync()
+    vol.synchronize()
+    X, Y, Z = (8 * c for c in sc.map_size)
+    W, H = sc.width, sc.height
+    out = DeviceBuffer(W * H * 16)
+    ctrl = [(1.0 + 14.0 * float(uniform01(k, 0, seed=4)), 7.0 + 6.0 * float(uniform01(k, 1, seed=4)),
+             1.0 + 14.0 * float(uniform01(k, 2, seed=4))) for k in range(8)]
+    n_mats = len(sc.materials) - 1
+    frames = args.frames
+    times, edit_ms, up_bytes, up_chunks, lit = [], [], [], [], []
+    vol.counters(KIND_LIGHTING, reset=True)
+    vol.lib.DNX_get_upload_stats(vol.ptr, None, None, None, True)
+    b, c_, v_ = (ctypes_u64() for _ in range(3))
+    for f in range(frames + args.warmup):
+        pos, tan = catmull_rom(ctrl, f / max(frames, 1))
+        vol.set_vec("camPos", tuple(float(x) for x in pos))
+        yaw = float(np.degrees(np.arctan2(tan[0], tan[2])))
+        vol.set_vec("camOrient", (25.0, yaw, 0.0))
+        t0 = time.perf_counter()
+        for k in range(256):
+            u = [float(uniform01(k, j, seed=4 + f)) for j in range(6)]
+            r = 3.0 + 21.0 * u[0]
+            ctr = (u[1] * X, u[2] * Y, u[3] * Z)
+            if u[4] < 0.5:
+                vol.shape_sphere(255, (0, 0, 0), ctr, r)
+            else:
+                m = min(n_mats - 1, int(u[5] * n_mats))
+                vol.shape_sphere(m, sc.materials[m].albedo, ctr, r)
+        t1 = time.perf_counter()
+        view, proj = vol.view_projection(H / W)
+        vol.draw(out, W, H, view, proj)
+        vol.sync()
+        vol.update_lighting(1, 1000, 123.456 + f / 60.0)
+        vol.synchronize()
+        t2 = time.perf_counter()
+        vol.lib.DNX_get_upload_stats(vol.ptr, b, c_, v_, True)
+        litf = vol.counters(KIND_LIGHTING, reset=True)["litVoxels"]
+        if f >= args.warmup:
+            times.append(t2 - t0)
+            edit_ms.append(1e3 * (t1 - t0))
+            up_bytes.append(b.value)
+            up_chunks.append(c_.value)
+            lit.append(litf)
+        if (f + 1) % 100 == 0:
+            print(f"[bench] {f + 1} frames", file=sys.stderr, flush=True)
+    t = np.array(times)
+    result = {
+        "metric": "dynamic edit streaming: voxel-lighting updates/s and frame time p50/p99",
+        "value": float(np.sum(lit) / np.sum(t)), "unit": "updates/s", "n_gpus": 1, "steps": frames,
+        "warmup": args.warmup, "ms_per_step": float(1e3 * t.mean()), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "16x16x16 chunks (128^3 voxels) fBm ~35%, 256 sphere edits/frame (radius 3-24, "
+                               "carve/fill 50/50), 256x256 draw, 16 direct/specular rays, 1 diffuse sample",
+                   "frames": frames, "parallelism": "single GPU"},
+        "frame_ms_p50": float(1e3 * np.percentile(t, 50)), "frame_ms_p99": float(1e3 * np.percentile(t, 99)),
+        "edit_ms_p50": float(np.percentile(edit_ms, 50)),
+        "upload_bytes_per_frame": float(np.mean(up_bytes)), "uploaded_chunks_per_frame": float(np.mean(up_chunks)),
+        "dirty_fraction": float(np.mean(up_chunks) / (sc.map_size[0] * sc.map_size[1] * sc.map_size[2])),
+        "voxels_lit_per_frame": float(np.mean(lit)),
+    }
+    print(json.dumps(result), flush=True)
+    vol.close()
+    return 0
+
+
+def bench_cache(args) -> int:
+    """Learned radiance cache vs plain accumulation on the 16^3-chunk fBm map
+    (emission 1, so irradiance stays in the stored [0, 1] range), every exposed
+    chunk lit each frame. Reference: the FP64 mean of `--ref-frames` frames of
+    the engine's own per-voxel samples, on a separate volume. Then, on a fresh
+    volume, each frame: plain lighting pass (running mean) and one cache
+    training step on 2^16 path-traced samples; both estimates are scored by
+    relative RMSE (sqrt(mean sq. error) / mean reference, over voxels and RGB)."""
+    import numpy as np
+
+    from doonengine_amd import scenes
+    from doonengine_amd._hip import DeviceBuffer
+    from doonengine_amd.volume import RadianceCache, Volume
+    from oracle.radiance_cache import relative_rmse
+
+    sc = scenes.config1(emission=1.0)
+
+    def make():
+        v = Volume(sc.map_size, min_chunks=4096)
+        sc.apply(v)
+        v.request_all()
+        v.mark_visible(1 << 20)
+        v.sync()
+        v.synchronize()
+        return v
+
+    cap = int(np.prod(sc.map_size)) * 16
+    slots = cap * 32
+    fresh = DeviceBuffer(slots * 16)
+    acc = DeviceBuffer(slots * 32)
+    acc.zero()
+    vref = make()
+    vref.lib.DNX_set_lighting_sample_output(vref.ptr, fresh.ptr, cap)
+    R, F = args.ref_frames, args.frames
+    t0 = time.perf_counter()
+    for f in range(R):
+        vref.mark_visible(1 << 20)
+        vref.sync()
+        vref.update_lighting(1, 1000, 500.0 + f / 60.0)
+        vref.lib.DNX_accumulate_f64(vref.ptr, fresh.ptr, acc.ptr, slots * 4)
+    vref.synchronize()
+    n = vref.fetch_requests().size
+    ref = acc.download(np.float64, (slots, 4))[:n * 32]
+    print(f"[bench] reference: {R} frames in {time.perf_counter() - t0:.1f} s, {n} requests", file=sys.stderr,
+          flush=True)
+    vref.lib.DNX_set_lighting_sample_output(vref.ptr, None, 0)
+    vref.close()
+    valid = ref[:, 3] == R
+    refm = np.clip(ref[valid, :3] / R, 0.0, 1.0)
+
+    vol = make()
+    dbg = DeviceBuffer(slots * 24)
+    idx = DeviceBuffer(slots * 4)
+    pred = DeviceBuffer(slots * 16)
+    vol.lib.DNX_set_lighting_debug_output(vol.ptr, dbg.ptr, idx.ptr, cap)
+    rc = RadianceCache(vol, samplesPerFrame=args.cache_samples, seed=1)
+    curve_p, curve_c = [], []
+    t_train = t_pred = 0.0
+    for f in range(F):
+        t = 123.456 + f / 60.0
+        vol.mark_visible(1 << 20)
+        vol.sync()
+        vol.update_lighting(1, 1000, t)
+        vol.synchronize()
+        a = time.perf_counter()
+        rc.train_frame(t, 1)
+        vol.synchronize()
+        b = time.perf_counter()
+        rc.predict_requests(pred, n)
+        vol.synchronize()
+        c = time.perf_counter()
+        t_train += b - a
+        t_pred += c - b
+        plain = dbg.download(np.float32, (slots, 6))[:n * 32, :3][valid]
+        cache = pred.download(np.float32, (slots, 4))[:n * 32, :3][valid]
+        curve_p.append(relative_rmse(np.clip(plain, 0, 1), refm))
+        curve_c.append(relative_rmse(np.clip(cache, 0, 1), refm))
+        if (f + 1) % 64 == 0:
+            print(f"[bench] frame {f + 1}: rel RMSE plain {curve_p[-1]:.4f} cache {curve_c[-1]:.4f}", file=sys.stderr,
+                  flush=True)
+    rc.close()
+    vol.lib.DNX_set_lighting_debug_output(vol.ptr, None, None, 0)
+    vol.close()
+
+    def first_below(curve, thr=0.01):
+        for i, e in enumerate(curve):
+            if e <= thr:
+                return i + 1
+        return None
+
+    fp, fc = first_below(curve_p), first_below(curve_c)
+    marks = [m for m in (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048) if m <= F]
+    result = {
+        "metric": "frames to 1% relative RMSE: radiance cache vs plain accumulation",
+        "value": fc, "unit": "frames", "n_gpus": 1, "steps": F, "warmup": 0,
+        "ms_per_step": 1e3 * (t_train + t_pred) / F, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 (MLP operands), f32 accumulate", "data": "synthetic",
+        "config": {"workload": "16x16x16 chunks (128^3 voxels) fBm ~35%, every exposed chunk lit per frame, "
+                               f"16 direct rays, 1 diffuse sample; cache: 16-level hash grid (2^19), MLP 3x64, "
+                               f"Adam lr 1e-2, {args.cache_samples} samples/frame; reference {R} frames FP64",
+                   "voxels": int(valid.sum()), "parallelism": "single GPU"},
+        "frames_to_1pct": {"cache": fc, "plain": fp},
+        "rel_rmse_at": {str(m): {"plain": curve_p[m - 1], "cache": curve_c[m - 1]} for m in marks},
+        "train_ms_per_frame": 1e3 * t_train / F, "predict_ms_per_frame": 1e3 * t_pred / F,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def ctypes_u64():
+    import ctypes
+    return ctypes.c_uint64()
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("frame", "batch"), default="frame",
-                    help="frame: the full-frame pipeline (default); batch: the batched uneven-volume lighting")
+    ap.add_argument("--workload", choices=("frame", "batch", "edits", "cache"), default="frame",
+                    help="frame: the full-frame pipeline (default); batch: the batched uneven-volume lighting; "
+                         "edits: dynamic edit streaming; cache: radiance cache vs plain accumulation")
     ap.add_argument("--scenes", type=int, default=384, help="volumes of the batch workload")
+    ap.add_argument("--frames", type=int, default=600, help="frames of the edits and cache workloads")
+    ap.add_argument("--ref-frames", type=int, default=4096, help="frames of the cache workload's FP64 reference")
+    ap.add_argument("--cache-samples", type=int, default=65536, help="cache training samples per frame")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -207,6 +422,8 @@ def main() -> int:
         return 2
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "edits":
+        return bench_edits(args)
 
     import numpy as np
 

@@ -171,6 +171,7 @@ SIGNATURES = {
     "DNX_set_draw_hit_output": (None, [POINTER(DNvolume), c_void_p, c_size_t]),
     "DNX_set_lighting_debug_output": (None, [POINTER(DNvolume), c_void_p, c_void_p, c_size_t]),
     "DNX_set_lighting_sample_output": (None, [POINTER(DNvolume), c_void_p, c_size_t]),
+    "DNX_get_upload_stats": (None, [POINTER(DNvolume), POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64), c_bool]),
     "DNX_draw_matrices": (None, [DNmat4, DNmat4, POINTER(DNmat4), POINTER(DNmat4)]),
     "DNX_batch_create": (c_void_p, [POINTER(POINTER(DNvolume)), c_uint32]),
     "DNX_batch_destroy": (None, [c_void_p]),

@@ -43,12 +43,34 @@ __device__ void shadow_ray(LCtx& L, f3 p, uint32_t sample, f3& color)
         return;
     }
     f3 acc = splat(0.0f);
+#ifdef DN_LIGHT_WW
+    // Each lane walks its own sequence of the fixed shadow rays, one
+    // chunk-grid trip per iteration, starting ray k + 1 as soon as ray k ends.
+    {
+        RayCtx& c = *L.c;
+        MapWalk w;
+        uint32_t k = 0;
+        walk_start(w, c, ld3(P.directRays[0]), ld3(P.directRays[0] + 3), p, true);
+        while(k < P.numDirectRays)
+        {
+            const int r = walk_trip(P.scene, c, w, hv);
+            if(r != 0)
+            {
+                walk_finish(c, r == 1);
+                if(r == 2) acc = acc + (L.sunStrength * w.cm + w.ca);
+                k++;
+                if(k < P.numDirectRays) walk_start(w, c, ld3(P.directRays[k]), ld3(P.directRays[k] + 3), p, true);
+            }
+        }
+    }
+#else
     for(uint32_t k = 0; k < P.numDirectRays; k++)
     {
         f3 pp = p;
         if(!light_trace(P.scene, *L.c, ld3(P.directRays[k]), ld3(P.directRays[k] + 3), pp, hv, ca, cm))
             acc = acc + (L.sunStrength * cm + ca);
     }
+#endif
     color = color + acc / (float)P.numDirectRays;
 }
 

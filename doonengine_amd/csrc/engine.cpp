@@ -191,6 +191,7 @@ struct DNgpuVolume {
     size_t dbgCap = 0;
     float* freshOut = nullptr;
     size_t freshCap = 0;
+    uint64_t upBytes = 0, upChunks = 0, upVoxels = 0;   // host-to-device sync traffic
 };
 
 namespace {
@@ -926,6 +927,9 @@ extern "C" void DN_sync_gpu(DNvolume* vol, DNmemOp op, int lightingSplit)
                 memcpy(h + oOcc + sizeof(uint32_t) * g->occWords, g->occPresent.data(), sizeof(uint32_t) * g->occWords);
                 Timer tm = timer_begin(g, TK_UPLOAD);
                 hipMemcpyAsync(g->dStage, h, bytes, hipMemcpyHostToDevice, g->stream);
+                g->upBytes += bytes;
+                g->upChunks += recs.size();
+                g->upVoxels += totalVox;
                 hipMemcpyAsync(g->dOccLoaded, g->dStage + oOcc, sizeof(uint32_t) * g->occWords, hipMemcpyDeviceToDevice,
                                g->stream);
                 hipMemcpyAsync(g->dOccPresent, g->dStage + oOcc + sizeof(uint32_t) * g->occWords,
@@ -950,6 +954,7 @@ extern "C" void DN_sync_gpu(DNvolume* vol, DNmemOp op, int lightingSplit)
                 memcpy(h + oOcc, g->occLoaded.data(), sizeof(uint32_t) * g->occWords);
                 memcpy(h + oOcc + sizeof(uint32_t) * g->occWords, g->occPresent.data(), sizeof(uint32_t) * g->occWords);
                 hipMemcpyAsync(g->dStage, h, bytes, hipMemcpyHostToDevice, g->stream);
+                g->upBytes += bytes;
                 hipMemcpyAsync(g->dOccLoaded, g->dStage + oOcc, sizeof(uint32_t) * g->occWords, hipMemcpyDeviceToDevice,
                                g->stream);
                 hipMemcpyAsync(g->dOccPresent, g->dStage + oOcc + sizeof(uint32_t) * g->occWords,
@@ -1777,6 +1782,15 @@ extern "C" void DNX_set_lighting_debug_output(DNvolume* vol, float* values, uint
     vol->gpu->dbgValues = values;
     vol->gpu->dbgIndex = voxelIndices;
     vol->gpu->dbgCap = requestCapacity;
+}
+
+extern "C" void DNX_get_upload_stats(DNvolume* vol, uint64_t* bytes, uint64_t* chunks, uint64_t* voxels, bool reset)
+{
+    DNgpuVolume* g = vol->gpu;
+    if(bytes) *bytes = g->upBytes;
+    if(chunks) *chunks = g->upChunks;
+    if(voxels) *voxels = g->upVoxels;
+    if(reset) g->upBytes = g->upChunks = g->upVoxels = 0;
 }
 
 extern "C" void DNX_set_lighting_sample_output(DNvolume* vol, float* samples, size_t requestCapacity)

@@ -208,11 +208,12 @@ def config_small(seed: int = 11, chunks=(4, 3, 4), occupancy: float = 0.3, n_box
                  cam_orient=(50.0, 45.0, 0.0), width=width, height=height)
 
 
-def config1(reference_like: bool = False) -> Scene:
+def config1(reference_like: bool = False, emission: float | None = None) -> Scene:
     """16x16x16 chunks (128^3 voxels), fBm terrain at ~35% occupancy, 4 materials
-    (one emissive), 256x256 camera; 16 direct/specular rays, 1 diffuse sample."""
-    nw, aw, mats = terrain((128, 128, 128), 0.35, seed=1, amplitude=48.0, n_materials=4,
-                           emission=0.0 if reference_like else 4.0)
+    (one emissive, strength 4 unless `emission` is given), 256x256 camera;
+    16 direct/specular rays, 1 diffuse sample."""
+    em = emission if emission is not None else (0.0 if reference_like else 4.0)
+    nw, aw, mats = terrain((128, 128, 128), 0.35, seed=1, amplitude=48.0, n_materials=4, emission=em)
     params = {} if reference_like else {"numSpecularRays": 16, "numDirectRays": 16}
     return Scene("config1", (16, 16, 16), nw, aw, mats, cam_pos=(1.0, 8.0, 1.0), cam_orient=(35.0, 45.0, 0.0),
                  width=256, height=256, params=params)

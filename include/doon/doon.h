@@ -328,6 +328,11 @@ void DNX_set_lighting_debug_output(DNvolume* vol, float* values, uint32_t* voxel
  * for an empty lane). NULL disables. */
 void DNX_set_lighting_sample_output(DNvolume* vol, float* samples, size_t requestCapacity);
 
+/* Host-to-device traffic of DN_sync_gpu since the last reset: staged bytes
+ * (upload ops, chunk records, voxels, occupancy bitmaps), chunks and voxels
+ * uploaded. The reference re-uploads through glBufferSubData (voxel.c:1491-1695). */
+void DNX_get_upload_stats(DNvolume* vol, uint64_t* bytes, uint64_t* chunks, uint64_t* voxels, bool reset);
+
 /* Camera matrices for the draw kernel, from DN_draw's arguments: the inverse
  * view matrix and invCenteredView * invProjection (voxel.c:844-853). */
 void DNX_draw_matrices(DNmat4 view, DNmat4 projection, DNmat4* invView, DNmat4* invViewProj);

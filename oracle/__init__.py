@@ -99,6 +99,13 @@ class Oracle:
         self.lib.or_decompress(w.ctypes.data_as(c_void_p), out.ctypes.data_as(c_void_p))
         return out
 
+    def set_fresh_output(self, out: np.ndarray | None) -> None:
+        """With `out` ((>= numRequests*32, 3) array of this oracle's real type,
+        kept alive by the caller), lighting() also writes each voxel's diffuse
+        samples of the pass, averaged, before the running mean. None disables."""
+        self._fresh = out
+        self.lib.or_set_fresh_output(None if out is None else out.ctypes.data_as(c_void_p))
+
     def set_rand_override(self, seeds=None, values=None) -> None:
         """Use `values` as the hash of exactly these `seeds` (tests only); None clears."""
         if seeds is None:

@@ -756,6 +756,13 @@ static inline uint32_t rne_u(real x) { return (uint32_t)r_rint(x); }
  * rays from visible chunks; sample count +1 on lit chunks.
  * preq (optional, numRequests*32*6) receives the pre-quantisation diffuse and
  * specular values, voxOf (optional, numRequests*32) the pool index per lane. */
+/* Optional output of or_lighting: numRequests*32*3 reals receiving each voxel's
+ * diffuse samples of the pass averaged before the running mean
+ * (voxelLighting.comp:254-261); lanes without a diffuse estimate are left as
+ * they were. NULL disables. */
+static real* g_freshOut = NULL;
+void or_set_fresh_output(real* out) { g_freshOut = out; }
+
 void or_lighting(const ORscene* s, const ORlightParams* lp, const uint32_t* requests, uint32_t numRequests,
                  uint32_t* voxelsOut, uint32_t* mapOut, uint32_t* chunksOut, real* preq, uint32_t* voxOf,
                  ORcounters* cnt)
@@ -820,6 +827,11 @@ void or_lighting(const ORscene* s, const ORlightParams* lp, const uint32_t* requ
                     diff = vadd(diff, amb);
                     diffuse_ray(&c, vadds(me.normal, EPS), p, me, (real)lp->time * (real)(i + 1), &diff);
                     shadow_ray(&c, p, (real)lp->time * (real)(i + 1 + lp->numDiffuseSamples), &diff);
+                }
+                if(g_freshOut && lp->numDiffuseSamples)
+                {
+                    v3 fr = vdivs(diff, (real)lp->numDiffuseSamples);
+                    g_freshOut[slot * 3 + 0] = fr.x; g_freshOut[slot * 3 + 1] = fr.y; g_freshOut[slot * 3 + 2] = fr.z;
                 }
                 diff = vdivs(vadd(vmuls(me.diffuse, indirect), diff), indirect + (real)lp->numDiffuseSamples);
             }
