@@ -287,11 +287,13 @@ def bench_edits(args) -> int:
 def bench_cache(args) -> int:
     """Learned radiance cache vs plain accumulation on the 16^3-chunk fBm map
     (emission 1, so irradiance stays in the stored [0, 1] range), every exposed
-    chunk lit each frame. Reference: the FP64 mean of `--ref-frames` frames of
-    the engine's own per-voxel samples, on a separate volume. Then, on a fresh
-    volume, each frame: plain lighting pass (running mean) and one cache
-    training step on 2^16 path-traced samples; both estimates are scored by
-    relative RMSE (sqrt(mean sq. error) / mean reference, over voxels and RGB)."""
+    chunk lit each frame. Reference: per voxel, the FP64 mean of `--ref-samples`
+    independent samples (per-sample random numbers, exact ambient + direct
+    term), so that its own noise stays well below 1%. Each frame: the plain
+    lighting pass (the engine's running mean, with the shader's frame-uniform
+    random numbers) and one cache training step on 2^16 path-traced samples;
+    both estimates are scored by relative RMSE (sqrt(mean squared error) / mean
+    reference, over voxels and RGB)."""
     import numpy as np
 
     from doonengine_amd import scenes
@@ -300,56 +302,58 @@ def bench_cache(args) -> int:
     from oracle.radiance_cache import relative_rmse
 
     sc = scenes.config1(emission=1.0)
-
-    def make():
-        v = Volume(sc.map_size, min_chunks=4096)
-        sc.apply(v)
-        v.request_all()
-        v.mark_visible(1 << 20)
-        v.sync()
-        v.synchronize()
-        return v
-
-    cap = int(np.prod(sc.map_size)) * 16
-    slots = cap * 32
-    fresh = DeviceBuffer(slots * 16)
+    vol = Volume(sc.map_size, min_chunks=4096)
+    sc.apply(vol)
+    vol.request_all()
+    vol.sync()                      # loads every chunk
+    vol.mark_visible(1 << 20)
+    vol.sync()                      # the lighting request list of every exposed chunk
+    vol.synchronize()
+    n = vol.fetch_requests().size
+    slots = n * 32
+    rc = RadianceCache(vol, samplesPerFrame=args.cache_samples, seed=1, lr=args.cache_lr, emaDecay=args.cache_ema,
+                       nMax=args.cache_nmax)
+    R, F = args.ref_samples, args.frames
     acc = DeviceBuffer(slots * 32)
     acc.zero()
-    vref = make()
-    vref.lib.DNX_set_lighting_sample_output(vref.ptr, fresh.ptr, cap)
-    R, F = args.ref_frames, args.frames
     t0 = time.perf_counter()
-    for f in range(R):
-        vref.mark_visible(1 << 20)
-        vref.sync()
-        vref.update_lighting(1, 1000, 500.0 + f / 60.0)
-        vref.lib.DNX_accumulate_f64(vref.ptr, fresh.ptr, acc.ptr, slots * 4)
-    vref.synchronize()
-    n = vref.fetch_requests().size
-    ref = acc.download(np.float64, (slots, 4))[:n * 32]
-    print(f"[bench] reference: {R} frames in {time.perf_counter() - t0:.1f} s, {n} requests", file=sys.stderr,
-          flush=True)
-    vref.lib.DNX_set_lighting_sample_output(vref.ptr, None, 0)
-    vref.close()
-    valid = ref[:, 3] == R
-    refm = np.clip(ref[valid, :3] / R, 0.0, 1.0)
+    rc.reference(acc, n, R, seed=7)
+    vol.synchronize()
+    ref = acc.download(np.float64, (slots, 4))
+    print(f"[bench] reference: {R} samples per voxel in {time.perf_counter() - t0:.1f} s, {n} requests",
+          file=sys.stderr, flush=True)
+    valid = ref[:, 3] > 0
+    refm = np.clip(ref[valid, :3] / ref[valid, 3:4], 0.0, 1.0)
 
-    vol = make()
     dbg = DeviceBuffer(slots * 24)
     idx = DeviceBuffer(slots * 4)
     pred = DeviceBuffer(slots * 16)
-    vol.lib.DNX_set_lighting_debug_output(vol.ptr, dbg.ptr, idx.ptr, cap)
-    rc = RadianceCache(vol, samplesPerFrame=args.cache_samples, seed=1)
-    curve_p, curve_c = [], []
+    vol.lib.DNX_set_lighting_debug_output(vol.ptr, dbg.ptr, idx.ptr, n)
+    # A second, cold accumulator started at frame F0: what an edit or a newly
+    # visible region gets (its lighting history restarts), against the warm cache.
+    F0 = F // 2
+    cold = Volume(sc.map_size, min_chunks=4096)
+    sc.apply(cold)
+    cold.request_all()
+    cold.sync()
+    dbg2, idx2 = DeviceBuffer(slots * 24), DeviceBuffer(slots * 4)
+    cold.lib.DNX_set_lighting_debug_output(cold.ptr, dbg2.ptr, idx2.ptr, n)
+    curve_p, curve_c, curve_cold = [], [], []
     t_train = t_pred = 0.0
     for f in range(F):
         t = 123.456 + f / 60.0
         vol.mark_visible(1 << 20)
         vol.sync()
         vol.update_lighting(1, 1000, t)
+        if f >= F0:
+            cold.mark_visible(1 << 20)
+            cold.sync()
+            cold.update_lighting(1, 1000, t)
+            cold.synchronize()
+            curve_cold.append(relative_rmse(np.clip(dbg2.download(np.float32, (slots, 6))[:, :3][valid], 0, 1), refm))
         vol.synchronize()
         a = time.perf_counter()
-        rc.train_frame(t, 1)
+        rc.train_frame(t, args.cache_steps)
         vol.synchronize()
         b = time.perf_counter()
         rc.predict_requests(pred, n)
@@ -358,21 +362,27 @@ def bench_cache(args) -> int:
         t_train += b - a
         t_pred += c - b
         plain = dbg.download(np.float32, (slots, 6))[:n * 32, :3][valid]
-        cache = pred.download(np.float32, (slots, 4))[:n * 32, :3][valid]
+        cache = np.clip(pred.download(np.float32, (slots, 4))[:n * 32, :3][valid], 0, 1).astype(np.float64)
         curve_p.append(relative_rmse(np.clip(plain, 0, 1), refm))
-        curve_c.append(relative_rmse(np.clip(cache, 0, 1), refm))
-        if (f + 1) % 64 == 0:
+        curve_c.append(relative_rmse(cache, refm))
+        if (f + 1) % 32 == 0 or f < 8:
             print(f"[bench] frame {f + 1}: rel RMSE plain {curve_p[-1]:.4f} cache {curve_c[-1]:.4f}", file=sys.stderr,
                   flush=True)
     rc.close()
     vol.lib.DNX_set_lighting_debug_output(vol.ptr, None, None, 0)
     vol.close()
+    cold.lib.DNX_set_lighting_debug_output(cold.ptr, None, None, 0)
+    cold.close()
 
     def first_below(curve, thr=0.01):
         for i, e in enumerate(curve):
             if e <= thr:
                 return i + 1
         return None
+
+    # frames the cold accumulator needs to match the warm cache's error at F0
+    warm = curve_c[F0 - 1] if F0 > 0 else None
+    catch_up = first_below(curve_cold, warm) if warm is not None else None
 
     fp, fc = first_below(curve_p), first_below(curve_c)
     marks = [m for m in (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048) if m <= F]
@@ -383,10 +393,17 @@ def bench_cache(args) -> int:
         "vs_baseline": None, "dtype": "bf16 (MLP operands), f32 accumulate", "data": "synthetic",
         "config": {"workload": "16x16x16 chunks (128^3 voxels) fBm ~35%, every exposed chunk lit per frame, "
                                f"16 direct rays, 1 diffuse sample; cache: 16-level hash grid (2^19), MLP 3x64, "
-                               f"Adam lr 1e-2, {args.cache_samples} samples/frame; reference {R} frames FP64",
+                               f"Adam lr {args.cache_lr}, {args.cache_steps} step(s) on {args.cache_samples} "
+                               f"samples/frame, predictions from the parameter average (decay {args.cache_ema}); "
+                               f"reference: {R} independent samples per voxel, FP64",
                    "voxels": int(valid.sum()), "parallelism": "single GPU"},
         "frames_to_1pct": {"cache": fc, "plain": fp},
         "rel_rmse_at": {str(m): {"plain": curve_p[m - 1], "cache": curve_c[m - 1]} for m in marks},
+        "after_history_reset": {
+            "reset_at_frame": F0, "warm_cache_rel_rmse": warm,
+            "cold_plain_rel_rmse_after": {str(k): curve_cold[k - 1] for k in (1, 2, 4, 8, 16, 32, 64, 128)
+                                          if k <= len(curve_cold)},
+            "frames_for_cold_plain_to_match_warm_cache": catch_up},
         "train_ms_per_frame": 1e3 * t_train / F, "predict_ms_per_frame": 1e3 * t_pred / F,
     }
     print(json.dumps(result), flush=True)
@@ -408,8 +425,13 @@ def main() -> int:
                          "edits: dynamic edit streaming; cache: radiance cache vs plain accumulation")
     ap.add_argument("--scenes", type=int, default=384, help="volumes of the batch workload")
     ap.add_argument("--frames", type=int, default=600, help="frames of the edits and cache workloads")
-    ap.add_argument("--ref-frames", type=int, default=4096, help="frames of the cache workload's FP64 reference")
+    ap.add_argument("--ref-samples", type=int, default=4096,
+                    help="independent samples per voxel of the cache workload's FP64 reference")
     ap.add_argument("--cache-samples", type=int, default=65536, help="cache training samples per frame")
+    ap.add_argument("--cache-steps", type=int, default=1, help="cache Adam steps per frame")
+    ap.add_argument("--cache-lr", type=float, default=1e-2, help="cache Adam learning rate")
+    ap.add_argument("--cache-ema", type=float, default=0.95, help="decay of the cache's parameter average")
+    ap.add_argument("--cache-nmax", type=int, default=256, help="finest hash-grid resolution over the map")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -424,6 +446,8 @@ def main() -> int:
         return bench_batch(args)
     if args.workload == "edits":
         return bench_edits(args)
+    if args.workload == "cache":
+        return bench_cache(args)
 
     import numpy as np
 

@@ -115,7 +115,7 @@ class DNXbatchConfig(Structure):
 class DNXrcConfig(Structure):
     _fields_ = [("log2Table", c_uint32), ("nMin", c_uint32), ("nMax", c_uint32), ("extent", c_float),
                 ("lr", c_float), ("beta1", c_float), ("beta2", c_float), ("eps", c_float),
-                ("samplesPerFrame", c_uint32), ("seed", c_uint32)]
+                ("samplesPerFrame", c_uint32), ("seed", c_uint32), ("emaDecay", c_float)]
 
 
 MSG_CB = ctypes.CFUNCTYPE(None, c_int, c_int, c_char_p)
@@ -191,6 +191,7 @@ SIGNATURES = {
     "DNX_rc_set_params": (c_bool, [c_void_p, c_void_p, c_void_p]),
     "DNX_rc_get_params": (c_bool, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "DNX_rc_last_samples": (c_void_p, [c_void_p, POINTER(c_uint32)]),
+    "DNX_rc_reference": (c_bool, [c_void_p, c_void_p, c_size_t, c_uint32, c_uint32]),
     "DNX_accumulate_f64": (c_bool, [POINTER(DNvolume), c_void_p, c_void_p, c_size_t]),
 }
 

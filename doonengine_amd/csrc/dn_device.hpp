@@ -315,75 +315,6 @@ __device__ bool step_map(const DNsceneView& s, RayCtx& c, f3& dir, f3 inv, f3& p
     return hit;
 }
 
-// step_map<false> cut at chunk-grid trips, so that a lane can run its own
-// sequence of independent rays: a lane whose ray has ended starts its next one
-// while the other lanes continue. The operations per ray are exactly those of
-// step_map<false> (voxelShared.comp:421-475); only the loop is resumable.
-struct MapWalk {
-    DDA d;
-    f3 lsd, dir, inv, p;
-    f3 ca;
-    float cm;
-    bool ignoreFirst, requested;
-};
-
-__device__ __forceinline__ void walk_start(MapWalk& w, RayCtx& c, f3 dir, f3 inv, f3 p, bool ignoreFirst)
-{
-    w.dir = dir;
-    w.inv = inv;
-    w.p = p;
-    w.ignoreFirst = ignoreFirst;
-    w.requested = false;
-    w.ca = splat(0.0f);
-    w.cm = 1.0f;
-    w.lsd = splat(0.0f);
-    init_dda(dir, inv, p, w.d);
-    c.sMap = 0; c.sVox = 0; c.hitIndex = 0xFFFFFFFFu;
-}
-
-// One chunk-grid trip (with the whole chunk walk when the tile is loaded).
-// 0: the ray goes on, 1: hit (w.p = hit position), 2: left the map.
-__device__ __forceinline__ int walk_trip(const DNsceneView& s, RayCtx& c, MapWalk& w, DVoxel& vox)
-{
-    DDA& d = w.d;
-    if(!(d.px >= 0 && d.py >= 0 && d.pz >= 0 && d.px < (int32_t)s.mapX && d.py < (int32_t)s.mapY &&
-         d.pz < (int32_t)s.mapZ))
-        return 2;
-    c.sMap++;
-    const uint32_t idx = (uint32_t)d.px + s.mapX * ((uint32_t)d.py + s.mapY * (uint32_t)d.pz);
-    f3 tn = splat(0.0f);
-    if(occ_bit(s.occLoaded, idx))
-    {
-        if(s.lastUsed[idx] != 0u) s.lastUsed[idx] = 0u;
-        f3 upd = w.p + w.dir * (minc(w.lsd) - DN_EPS);
-        f3 crp = (upd - mk((float)d.px, (float)d.py, (float)d.pz)) * 8.0f;
-        crp = vclamp(crp, DN_EPS, 8.0f - DN_EPS);
-        bool refr;
-        if(step_chunk<false>(s, c, idx, w.dir, w.inv, crp, w.ignoreFirst, tn, vox, w.ca, w.cm, refr))
-        {
-            w.p = mk((float)d.px, (float)d.py, (float)d.pz) + crp * 0.125f;
-            return 1;
-        }
-    }
-    else if(s.hasUnloaded && !w.requested && occ_bit(s.occPresent, idx))
-    {
-        s.flags[idx] = 3u;
-        w.requested = true;
-    }
-    w.lsd = d.side;
-    iterate_dda(d, tn);
-    w.ignoreFirst = false;
-    return 0;
-}
-
-__device__ __forceinline__ void walk_finish(RayCtx& c, bool hit)
-{
-    c.cRays++;
-    c.cMap += c.sMap;
-    c.cVox += c.sVox;
-    c.cHits += hit ? 1u : 0u;
-}
-
 // step_map without refraction, as one loop over both DDA levels. The chunk
 // level and the map level share delta = |1/dir| and step = sign(dir), so the
 // trip ends with the same `lsd = side; iterate` for every lane whatever its

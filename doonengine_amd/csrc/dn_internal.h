@@ -181,6 +181,12 @@ struct DNrcNet {
     float* wGrad;
     float* wM;
     float* wV;
+    // Exponential moving average of the parameters over the Adam steps, and
+    // its debiased bf16 copies, which predictions use.
+    float* tabE;
+    uint32_t* tabE16;
+    float* wE;
+    uint16_t* wE16;
 };
 // Per-batch activation buffers of training, `tiles` = samples / 16. The *T
 // buffers are tile-transposed bf16: [tile][feature][16 samples].
@@ -195,6 +201,8 @@ struct DNrcActs {
 struct DNrcAdam {
     float lr, beta1, beta2, eps;
     float bias1, bias2;            // 1 - beta^t
+    float emaDecay;                // d: E = d * E + (1 - d) * W after each step
+    float emaScale;                // 1 / (1 - d^t): debiasing of E (which starts at 0)
 };
 
 int dn_rc_encode(const DNrcNet& net, const float* samples, uint32_t n, uint16_t* x0, hipStream_t stream);
@@ -207,6 +215,10 @@ int dn_rc_samples(const DNlightParams& p, uint32_t maxRequests, uint32_t n, uint
                   hipStream_t stream);
 int dn_rc_gather(const DNlightParams& p, uint32_t slots, float* samples, hipStream_t stream);
 int dn_accumulate_f64(const float* src, double* dst, size_t n, hipStream_t stream);
+// acc[slot] (4 doubles: rgb sum, count) += samplesPerVoxel (rounded up to the
+// work split) independent samples of every stored voxel of the request list.
+int dn_rc_reference(const DNlightParams& p, uint32_t slots, uint32_t samplesPerVoxel, uint32_t seed, double* acc,
+                    hipStream_t stream);
 
 // Fills the lighting-pass parameters of `vol` for a frame (engine.cpp).
 struct DNvolume;

@@ -20,8 +20,12 @@ __device__ __forceinline__ bool light_trace(const DNsceneView& s, RayCtx& c, f3 
 #endif
 }
 
+// Per-voxel context. `s` is the scene view every ray walks: the caller keeps it
+// in registers (a kernel argument, or a copy made once per work unit when the
+// parameters live in memory), so the DDA loops never reload it.
 struct LCtx {
     const DNlightParams* P;
+    const DNsceneView* s;
     RayCtx* c;
     bool firstSample;
     bool dry;                    // compute only: no stores, no bitmap updates
@@ -38,46 +42,24 @@ __device__ void shadow_ray(LCtx& L, f3 p, uint32_t sample, f3& color)
     {
         const float* r = L.firstSample ? P.firstShadow : P.shadowRays[sample];
         f3 pp = p;
-        if(!light_trace(P.scene, *L.c, ld3(r), ld3(r + 3), pp, hv, ca, cm))
+        if(!light_trace(*L.s, *L.c, ld3(r), ld3(r + 3), pp, hv, ca, cm))
             color = color + (L.sunStrength * cm + ca);
         return;
     }
     f3 acc = splat(0.0f);
-#ifdef DN_LIGHT_WW
-    // Each lane walks its own sequence of the fixed shadow rays, one
-    // chunk-grid trip per iteration, starting ray k + 1 as soon as ray k ends.
-    {
-        RayCtx& c = *L.c;
-        MapWalk w;
-        uint32_t k = 0;
-        walk_start(w, c, ld3(P.directRays[0]), ld3(P.directRays[0] + 3), p, true);
-        while(k < P.numDirectRays)
-        {
-            const int r = walk_trip(P.scene, c, w, hv);
-            if(r != 0)
-            {
-                walk_finish(c, r == 1);
-                if(r == 2) acc = acc + (L.sunStrength * w.cm + w.ca);
-                k++;
-                if(k < P.numDirectRays) walk_start(w, c, ld3(P.directRays[k]), ld3(P.directRays[k] + 3), p, true);
-            }
-        }
-    }
-#else
     for(uint32_t k = 0; k < P.numDirectRays; k++)
     {
         f3 pp = p;
-        if(!light_trace(P.scene, *L.c, ld3(P.directRays[k]), ld3(P.directRays[k] + 3), pp, hv, ca, cm))
+        if(!light_trace(*L.s, *L.c, ld3(P.directRays[k]), ld3(P.directRays[k] + 3), pp, hv, ca, cm))
             acc = acc + (L.sunStrength * cm + ca);
     }
-#endif
     color = color + acc / (float)P.numDirectRays;
 }
 
 __device__ void specular_ray(LCtx& L, uint32_t thisMapIndex, f3 p, f3 dir, f3 albedo, uint32_t reflectType, f3& color)
 {
     const DNlightParams& P = *L.P;
-    const DNsceneView& s = P.scene;
+    const DNsceneView& s = *L.s;
     DVoxel hv = {};
     f3 lastPos = p;
     f3 mult = albedo;
@@ -140,7 +122,7 @@ __device__ void diffuse_ray(LCtx& L, f3 normal, f3 p, const DVoxel& initial, uin
                             const RNG& rng)
 {
     const DNlightParams& P = *L.P;
-    const DNsceneView& s = P.scene;
+    const DNsceneView& s = *L.s;
     DVoxel hv = initial;
     f3 hitNormal = normal;
     float hitSpec = 0.0f;
@@ -189,9 +171,9 @@ __device__ void diffuse_ray(LCtx& L, f3 normal, f3 p, const DVoxel& initial, uin
 
 __device__ __forceinline__ uint32_t rne_u(float x) { return (uint32_t)rintf(x); }
 
-__device__ void light_voxel(const DNlightParams& P, uint32_t r, uint32_t lane32, RayCtx& c, uint32_t& lit, bool dry)
+__device__ void light_voxel(const DNlightParams& P, const DNsceneView& s, uint32_t r, uint32_t lane32, RayCtx& c,
+                            uint32_t& lit, bool dry)
 {
-    const DNsceneView& s = P.scene;
     const uint32_t req = P.requests[r];
     const uint32_t mapIndex = req >> 4, block = req & 15u;
     const uint32_t voxNum = lane32 + block * 32u;
@@ -217,7 +199,7 @@ __device__ void light_voxel(const DNlightParams& P, uint32_t r, uint32_t lane32,
     const uint32_t ns = rec->samples;
     const float indirect = (float)(ns < P.maxDiffuseSamples ? ns : P.maxDiffuseSamples);
     LCtx L;
-    L.P = &P; L.c = &c; L.dry = dry;
+    L.P = &P; L.s = &s; L.c = &c; L.dry = dry;
     L.firstSample = indirect == 0.0f;
     L.sunDir = ld3(P.sunDir);
     L.sunStrength = ld3(s.sunStrength);

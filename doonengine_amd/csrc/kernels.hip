@@ -132,7 +132,7 @@ __global__ __launch_bounds__(256) void k_lighting(DNlightParams P)
         base = __shfl(base, 0, 64);
         if(base >= nreq) break;
         const uint32_t r = base + (lane >> 5);
-        if(r < nreq) light_voxel(P, r, lane & 31u, c, lit, false);
+        if(r < nreq) light_voxel(P, P.scene, r, lane & 31u, c, lit, false);
     }
     if(P.counters)
     {
@@ -414,7 +414,7 @@ __device__ __forceinline__ void flush_counters(uint64_t* cnt, RayCtx& c, uint32_
 // queues the home queue of workgroup b is b % 8 and a volume's units stay on
 // one XCD (one L2) while its queue lasts.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_lighting_batch(DNbatchParams B)
+__global__ __launch_bounds__(BLOCK) void k_lighting_batch(DNbatchParams B, const DNlightParams* __restrict__ scenes)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t home = blockIdx.x % B.queues;
@@ -446,20 +446,21 @@ __global__ __launch_bounds__(BLOCK) void k_lighting_batch(DNbatchParams B)
             const uint32_t sid = __builtin_amdgcn_readfirstlane(B.queueScenes[(size_t)q * B.numScenes + lo]);
             if(sid != cur)
             {
-                if(cur != 0xFFFFFFFFu && !dry) flush_counters(B.scenes[cur].counters, c, lit);
+                if(cur != 0xFFFFFFFFu && !dry) flush_counters(scenes[cur].counters, c, lit);
                 cur = sid;
             }
-            const DNlightParams& P = B.scenes[sid];
+            const DNlightParams& P = scenes[sid];
+            const DNsceneView sv = P.scene;             // once per unit, into registers
             const uint32_t nreq = *P.numRequests;
             const uint32_t r0 = (u - pre[lo]) * B.unit;
             for(uint32_t j = 0; j < B.unit; j += 2u)
             {
                 const uint32_t r = r0 + j + (lane >> 5);
-                if(r < nreq) light_voxel(P, r, lane & 31u, c, lit, dry);
+                if(r < nreq) light_voxel(P, sv, r, lane & 31u, c, lit, dry);
             }
         }
     }
-    if(cur != 0xFFFFFFFFu && !dry) flush_counters(B.scenes[cur].counters, c, lit);
+    if(cur != 0xFFFFFFFFu && !dry) flush_counters(scenes[cur].counters, c, lit);
 }
 
 // Copy-back and epilogue of every volume of a batch: blockIdx.y = volume.
@@ -557,8 +558,8 @@ int dn_launch_batch(const DNbatchParams& b, uint32_t blockThreads, uint32_t bloc
     uint32_t per = (uint32_t)perCU;
     if(blocksPerCU && blocksPerCU < per) per = blocksPerCU;
     const uint32_t blocks = (uint32_t)cu_count() * per;
-    if(blockThreads == 64u) hipLaunchKernelGGL(k_lighting_batch<64>, dim3(blocks), dim3(64), 0, stream, b);
-    else hipLaunchKernelGGL(k_lighting_batch<256>, dim3(blocks), dim3(256), 0, stream, b);
+    if(blockThreads == 64u) hipLaunchKernelGGL(k_lighting_batch<64>, dim3(blocks), dim3(64), 0, stream, b, b.scenes);
+    else hipLaunchKernelGGL(k_lighting_batch<256>, dim3(blocks), dim3(256), 0, stream, b, b.scenes);
     if(epilogue && !b.dry)
     {
         uint32_t cx = (maxRequests * 32u + 255u) / 256u;

@@ -84,9 +84,11 @@ bool upload_params(DNXradianceCache* rc, const float* w, const float* tab)
                 ok(hipMemcpyAsync(n.w16, w16.data(), 2 * RC_NPARAM, hipMemcpyHostToDevice, rc->stream), "upload") &&
                 ok(hipMemcpyAsync(n.tab32, tab, sizeof(float) * tf, hipMemcpyHostToDevice, rc->stream), "upload") &&
                 ok(hipMemcpyAsync(n.tab16, t16.data(), 4 * (tf / 2), hipMemcpyHostToDevice, rc->stream), "upload");
-    for(float* p : {n.wGrad, n.wM, n.wV})
+    good = good && ok(hipMemcpyAsync(n.wE16, w16.data(), 2 * RC_NPARAM, hipMemcpyHostToDevice, rc->stream), "upload") &&
+           ok(hipMemcpyAsync(n.tabE16, t16.data(), 4 * (tf / 2), hipMemcpyHostToDevice, rc->stream), "upload");
+    for(float* p : {n.wGrad, n.wM, n.wV, n.wE})
         good = good && ok(hipMemsetAsync(p, 0, sizeof(float) * RC_NPARAM, rc->stream), "clear");
-    for(float* p : {n.tabGrad, n.tabM, n.tabV})
+    for(float* p : {n.tabGrad, n.tabM, n.tabV, n.tabE})
         good = good && ok(hipMemsetAsync(p, 0, sizeof(float) * tf, rc->stream), "clear");
     good = good && ok(hipStreamSynchronize(rc->stream), "upload");
     rc->adamStep = 0;
@@ -130,7 +132,19 @@ bool adam_step(DNXradianceCache* rc)
     o.eps = rc->cfg.eps;
     o.bias1 = (float)(1.0 - std::pow((double)rc->cfg.beta1, (double)rc->adamStep));
     o.bias2 = (float)(1.0 - std::pow((double)rc->cfg.beta2, (double)rc->adamStep));
+    o.emaDecay = rc->cfg.emaDecay;
+    o.emaScale = (float)(1.0 / (1.0 - std::pow((double)rc->cfg.emaDecay, (double)rc->adamStep)));
     return dn_rc_adam(rc->net, o, rc->stream) == 0;
+}
+
+// The parameters predictions read: the debiased moving average (whose bf16
+// copies equal the parameters themselves until the first Adam step).
+DNrcNet infer_net(const DNXradianceCache* rc)
+{
+    DNrcNet n = rc->net;
+    n.tab16 = rc->net.tabE16;
+    n.w16 = rc->net.wE16;
+    return n;
 }
 
 }  // namespace
@@ -148,6 +162,7 @@ extern "C" void DNX_rc_default_config(DNXrcConfig* c)
     c->eps = 1e-10f;
     c->samplesPerFrame = 65536;
     c->seed = 1;
+    c->emaDecay = 0.95f;
 }
 
 extern "C" void DNX_rc_destroy(DNXradianceCache* rc)
@@ -155,8 +170,8 @@ extern "C" void DNX_rc_destroy(DNXradianceCache* rc)
     if(!rc) return;
     if(rc->stream) hipStreamSynchronize(rc->stream);
     DNrcNet& n = rc->net;
-    dfree(n.tab16); dfree(n.tab32); dfree(n.tabGrad); dfree(n.tabM); dfree(n.tabV);
-    dfree(n.w16); dfree(n.w32); dfree(n.wGrad); dfree(n.wM); dfree(n.wV);
+    dfree(n.tab16); dfree(n.tab32); dfree(n.tabGrad); dfree(n.tabM); dfree(n.tabV); dfree(n.tabE); dfree(n.tabE16);
+    dfree(n.w16); dfree(n.w32); dfree(n.wGrad); dfree(n.wM); dfree(n.wV); dfree(n.wE); dfree(n.wE16);
     DNrcActs& a = rc->acts;
     dfree(rc->dSamples); dfree(a.x0); dfree(a.x0T); dfree(a.dyT); dfree(a.denc);
     for(int i = 0; i < 3; i++) { dfree(a.hT[i]); dfree(a.dzT[i]); }
@@ -172,11 +187,11 @@ extern "C" DNXradianceCache* DNX_rc_create(DNvolume* vol, const DNXrcConfig* con
     DNX_rc_default_config(&c);
     if(config) c = *config;
     if(c.log2Table < 4 || c.log2Table > 24 || c.nMin < 1 || c.nMax < c.nMin || (c.samplesPerFrame & 15u) ||
-       c.samplesPerFrame == 0 || !(c.lr > 0.0f))
+       c.samplesPerFrame == 0 || !(c.lr > 0.0f) || !(c.emaDecay >= 0.0f && c.emaDecay < 1.0f))
     {
         dn_msg(DN_MESSAGE_SHADER, DN_MESSAGE_ERROR,
                "radiance cache config: log2Table in [4, 24], 1 <= nMin <= nMax, samplesPerFrame a positive "
-               "multiple of 16, lr > 0");
+               "multiple of 16, lr > 0, emaDecay in [0, 1)");
         return nullptr;
     }
     DNXradianceCache* rc = new DNXradianceCache();
@@ -203,17 +218,20 @@ extern "C" DNXradianceCache* DNX_rc_create(DNvolume* vol, const DNXrcConfig* con
     }
     const size_t tf = table_floats(rc);
     bool g = dev(&n.tab16, tf / 2, "hash table") && dev(&n.tab32, tf, "hash table") && dev(&n.tabGrad, tf, "hash table") &&
-             dev(&n.tabM, tf, "hash table") && dev(&n.tabV, tf, "hash table") && dev(&n.w16, RC_NPARAM, "weights") &&
+             dev(&n.tabM, tf, "hash table") && dev(&n.tabV, tf, "hash table") && dev(&n.tabE, tf, "hash table") &&
+             dev(&n.tabE16, tf / 2, "hash table") && dev(&n.w16, RC_NPARAM, "weights") &&
              dev(&n.w32, RC_NPARAM, "weights") && dev(&n.wGrad, RC_NPARAM, "weights") &&
-             dev(&n.wM, RC_NPARAM, "weights") && dev(&n.wV, RC_NPARAM, "weights");
+             dev(&n.wM, RC_NPARAM, "weights") && dev(&n.wV, RC_NPARAM, "weights") && dev(&n.wE, RC_NPARAM, "weights") &&
+             dev(&n.wE16, RC_NPARAM, "weights");
     if(g)
     {
-        // Xavier-uniform weights on the used block of each padded matrix;
+        // Xavier-uniform weights on the used block of the hidden layers, a zero
+        // output layer (the first predictions are 0, not a random scale);
         // table entries uniform in [-1e-4, 1e-4]
         Mix rng{0x5DEECE66Dull ^ ((uint64_t)c.seed << 32)};
         std::vector<float> w(RC_NPARAM, 0.0f);
-        const struct { int off, ld, rows, cols; } mats[4] = {
-            {RC_OFF_W0, 48, 64, 38}, {RC_OFF_W1, 64, 64, 64}, {RC_OFF_W2, 64, 64, 64}, {RC_OFF_W3, 64, 3, 64}};
+        const struct { int off, ld, rows, cols; } mats[3] = {
+            {RC_OFF_W0, 48, 64, 38}, {RC_OFF_W1, 64, 64, 64}, {RC_OFF_W2, 64, 64, 64}};
         for(const auto& m : mats)
         {
             const double a = std::sqrt(6.0 / (double)(m.rows + m.cols));
@@ -265,8 +283,9 @@ extern "C" bool DNX_rc_predict_samples(DNXradianceCache* rc, const float* sample
 {
     if(!rc || !samples || !out || n == 0 || (n & 15u)) return false;
     if(!ensure_query(rc, n)) return false;
-    return dn_rc_encode(rc->net, samples, n, rc->dQueryX0, rc->stream) == 0 &&
-           dn_rc_forward(rc->net, rc->dQueryX0, samples, n, out, rc->stream) == 0;
+    const DNrcNet inf = infer_net(rc);
+    return dn_rc_encode(inf, samples, n, rc->dQueryX0, rc->stream) == 0 &&
+           dn_rc_forward(inf, rc->dQueryX0, samples, n, out, rc->stream) == 0;
 }
 
 extern "C" bool DNX_rc_train_frame(DNXradianceCache* rc, float time, int steps)
@@ -295,9 +314,22 @@ extern "C" bool DNX_rc_predict_requests(DNXradianceCache* rc, float* out, size_t
     size_t slots = (size_t)requestCapacity * 32u;
     if(slots == 0) return true;
     if(slots > 0xFFFFFFF0u || !ensure_query(rc, slots)) return false;
+    const DNrcNet inf = infer_net(rc);
     return dn_rc_gather(p, (uint32_t)slots, rc->dQuery, rc->stream) == 0 &&
-           dn_rc_encode(rc->net, rc->dQuery, (uint32_t)slots, rc->dQueryX0, rc->stream) == 0 &&
-           dn_rc_forward(rc->net, rc->dQueryX0, rc->dQuery, (uint32_t)slots, out, rc->stream) == 0;
+           dn_rc_encode(inf, rc->dQuery, (uint32_t)slots, rc->dQueryX0, rc->stream) == 0 &&
+           dn_rc_forward(inf, rc->dQueryX0, rc->dQuery, (uint32_t)slots, out, rc->stream) == 0;
+}
+
+extern "C" bool DNX_rc_reference(DNXradianceCache* rc, double* acc, size_t requestCapacity, uint32_t samplesPerVoxel,
+                                 uint32_t seed)
+{
+    if(!rc || !acc) return false;
+    DNlightParams p;
+    uint32_t maxReq = 0;
+    if(!prepare_lighting(rc->vol, 1, 1000, rc->vol->lastTime, p, maxReq)) return false;
+    const size_t slots = requestCapacity * 32u;
+    if(slots > 0x7FFFFFF0u / 16u) return false;
+    return dn_rc_reference(p, (uint32_t)slots, samplesPerVoxel, seed, acc, rc->stream) == 0;
 }
 
 extern "C" const float* DNX_rc_last_samples(DNXradianceCache* rc, uint32_t* n)

@@ -223,8 +223,11 @@ __global__ __launch_bounds__(256) void k_rc_mlp(DNrcNet net, const uint16_t* x0,
         {
             if(lane < 16u && sm < n)
             {
-                const float valid = samples ? samples[(size_t)sm * RC_SAMPLE + 12] : 1.0f;
-                reinterpret_cast<float4*>(pred)[sm] = make_float4(y[0], y[1], y[2], valid);
+                // a row may carry an exactly computed term (direct light) in 13..15
+                const float* sp = samples ? samples + (size_t)sm * RC_SAMPLE : nullptr;
+                const float valid = sp ? sp[12] : 1.0f;
+                const float b0 = sp ? sp[13] : 0.0f, b1 = sp ? sp[14] : 0.0f, b2 = sp ? sp[15] : 0.0f;
+                reinterpret_cast<float4*>(pred)[sm] = make_float4(y[0] + b0, y[1] + b1, y[2] + b2, valid);
             }
             continue;
         }
@@ -333,16 +336,24 @@ __device__ __forceinline__ float adam_one(float& w, float g, float& m, float& v,
     return w;
 }
 
+__device__ __forceinline__ float ema_one(float& e, float w, const DNrcAdam& o)
+{
+    e = o.emaDecay * e + (1.0f - o.emaDecay) * w;
+    return e * o.emaScale;
+}
+
 __global__ __launch_bounds__(256) void k_rc_adam(DNrcNet net, DNrcAdam o, size_t entries)
 {
     const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
     if(i < RC_NPARAM)
     {
-        float w = net.w32[i], m = net.wM[i], v = net.wV[i];
+        float w = net.w32[i], m = net.wM[i], v = net.wV[i], e = net.wE[i];
         adam_one(w, net.wGrad[i], m, v, o);
         net.w32[i] = w; net.wM[i] = m; net.wV[i] = v;
         net.wGrad[i] = 0.0f;
         net.w16[i] = f2bf(w);
+        net.wE16[i] = f2bf(ema_one(e, w, o));
+        net.wE[i] = e;
     }
     if(i < entries)
     {
@@ -350,6 +361,7 @@ __global__ __launch_bounds__(256) void k_rc_adam(DNrcNet net, DNrcAdam o, size_t
         float2 g = reinterpret_cast<float2*>(net.tabGrad)[i];
         float2 m = reinterpret_cast<float2*>(net.tabM)[i];
         float2 v = reinterpret_cast<float2*>(net.tabV)[i];
+        float2 e = reinterpret_cast<float2*>(net.tabE)[i];
         adam_one(w.x, g.x, m.x, v.x, o);
         adam_one(w.y, g.y, m.y, v.y, o);
         reinterpret_cast<float2*>(net.tab32)[i] = w;
@@ -357,6 +369,9 @@ __global__ __launch_bounds__(256) void k_rc_adam(DNrcNet net, DNrcAdam o, size_t
         reinterpret_cast<float2*>(net.tabV)[i] = v;
         reinterpret_cast<float2*>(net.tabGrad)[i] = make_float2(0.0f, 0.0f);
         net.tab16[i] = (uint32_t)f2bf(w.x) | ((uint32_t)f2bf(w.y) << 16);
+        const float ex = ema_one(e.x, w.x, o), ey = ema_one(e.y, w.y, o);
+        reinterpret_cast<float2*>(net.tabE)[i] = e;
+        net.tabE16[i] = (uint32_t)f2bf(ex) | ((uint32_t)f2bf(ey) << 16);
     }
 }
 
@@ -408,13 +423,52 @@ __device__ __forceinline__ bool slot_voxel(const DNlightParams& P, uint32_t slot
     return sel >= 0;
 }
 
-__device__ __forceinline__ void write_sample(float* out, f3 pos, const DVoxel& me, f3 target, float valid)
+__device__ __forceinline__ void write_sample(float* out, f3 pos, const DVoxel& me, f3 target, float valid,
+                                             f3 exact = f3{0.0f, 0.0f, 0.0f})
 {
     float4* o = reinterpret_cast<float4*>(out);
     o[0] = make_float4(pos.x, pos.y, pos.z, me.normal.x);
     o[1] = make_float4(me.normal.y, me.normal.z, me.albedo.x, me.albedo.y);
     o[2] = make_float4(me.albedo.z, target.x, target.y, target.z);
-    o[3] = make_float4(valid, 0.0f, 0.0f, 0.0f);
+    o[3] = make_float4(valid, exact.x, exact.y, exact.z);
+}
+
+// What the cache learns and what is computed exactly. With fixed direct rays
+// (numDirectRays > 0) the ambient and direct terms of a voxel's estimate
+// (voxelLighting.comp:255,260) are deterministic: they are evaluated exactly
+// and the network learns only the path-traced indirect term. With the
+// jittered shadow ray the whole estimate is noisy and the network learns all
+// of it.
+__device__ __forceinline__ bool direct_is_exact(const DNlightParams& P) { return P.numDirectRays != 0u; }
+
+struct VoxelSite {
+    const DNchunkRec* rec;
+    uint32_t voxNum;
+    int32_t sel;
+    DVoxel me;
+    f3 p;      // ray origin, voxelLighting.comp:231-232
+    f3 pos;    // voxel centre in voxel units
+};
+
+__device__ __forceinline__ VoxelSite voxel_site(const DNsceneView& s, const DNchunkRec* rec, uint32_t voxNum, int32_t sel)
+{
+    VoxelSite v;
+    v.rec = rec; v.voxNum = voxNum; v.sel = sel;
+    v.me = decompress(s.voxels[rec->voxelIndex + voxNum]);
+    const int32_t cx = sel % 8, cy = (sel / 8) % 8, cz = sel / 64;
+    v.p = (mk((float)cx, (float)cy, (float)cz) * 0.125f + mk((float)rec->pos[0], (float)rec->pos[1], (float)rec->pos[2])) +
+          splat(0.0625f);
+    v.p = v.p + splat(0.0625f - DN_EPS) * v.me.normal;
+    v.pos = mk((float)(rec->pos[0] * 8 + cx), (float)(rec->pos[1] * 8 + cy), (float)(rec->pos[2] * 8 + cz)) + splat(0.5f);
+    return v;
+}
+
+__device__ __forceinline__ void site_ctx(const DNlightParams& P, const DNsceneView& s, RayCtx& c, LCtx& L)
+{
+    ctx_init(c);
+    L.P = &P; L.s = &s; L.c = &c; L.dry = true; L.firstSample = false;
+    L.sunDir = ld3(P.sunDir);
+    L.sunStrength = ld3(s.sunStrength);
 }
 
 __global__ __launch_bounds__(256) void k_rc_samples(DNlightParams P, uint32_t n, uint32_t seed, float* samples)
@@ -436,39 +490,27 @@ __global__ __launch_bounds__(256) void k_rc_samples(DNlightParams P, uint32_t n,
         return;
     }
     const DNsceneView& s = P.scene;
-    const uint4 cw = s.voxels[rec->voxelIndex + voxNum];
-    const DVoxel me = decompress(cw);
-    const DNmat& mat = s.mats[me.material];
-    const int32_t cx = sel % 8, cy = (sel / 8) % 8, cz = sel / 64;
+    const VoxelSite v = voxel_site(s, rec, voxNum, sel);
+    const DNmat& mat = s.mats[v.me.material];
     RayCtx c;
-    ctx_init(c);
     LCtx L;
-    L.P = &P; L.c = &c; L.dry = true; L.firstSample = false;
-    L.sunDir = ld3(P.sunDir);
-    L.sunStrength = ld3(s.sunStrength);
-    f3 p = (mk((float)cx, (float)cy, (float)cz) * 0.125f + mk((float)rec->pos[0], (float)rec->pos[1], (float)rec->pos[2])) +
-           splat(0.0625f);
-    p = p + splat(0.0625f - DN_EPS) * me.normal;
+    site_ctx(P, s, c, L);
     f3 diff = splat(0.0f);
     if(mat.specular < 1.0f)
     {
         const SampleRandom rng{base};
-        diff = ld3(P.ambient);
-        diffuse_ray(L, me.normal + DN_EPS, p, me, 0, diff, rng);
-        if(P.numDirectRays)
-            shadow_ray(L, p, 0, diff);
-        else
+        diffuse_ray(L, v.me.normal + DN_EPS, v.p, v.me, 0, diff, rng);
+        if(!direct_is_exact(P))
         {
+            diff = diff + ld3(P.ambient);
             f3 dir = vnorm(L.sunDir * P.shadowSoftness + rng.ball(400u)) + DN_EPS;
             DVoxel hv = {};
             f3 ca; float cm;
-            f3 pp = p;
+            f3 pp = v.p;
             if(!light_trace(s, c, dir, vinv(dir), pp, hv, ca, cm)) diff = diff + (L.sunStrength * cm + ca);
         }
     }
-    const f3 pos = mk((float)(rec->pos[0] * 8 + cx), (float)(rec->pos[1] * 8 + cy), (float)(rec->pos[2] * 8 + cz)) +
-                   splat(0.5f);
-    write_sample(out, pos, me, diff, 1.0f);
+    write_sample(out, v.pos, v.me, diff, 1.0f);
 }
 
 // Query rows for every slot of the request list (valid 0 for empty lanes).
@@ -485,11 +527,72 @@ __global__ __launch_bounds__(256) void k_rc_gather(DNlightParams P, uint32_t slo
         write_sample(out, splat(0.0f), DVoxel{}, splat(0.0f), 0.0f);
         return;
     }
-    const DVoxel me = decompress(P.scene.voxels[rec->voxelIndex + voxNum]);
-    const int32_t cx = sel % 8, cy = (sel / 8) % 8, cz = sel / 64;
-    const f3 pos = mk((float)(rec->pos[0] * 8 + cx), (float)(rec->pos[1] * 8 + cy), (float)(rec->pos[2] * 8 + cz)) +
-                   splat(0.5f);
-    write_sample(out, pos, me, splat(0.0f), 1.0f);
+    const DNsceneView& s = P.scene;
+    const VoxelSite v = voxel_site(s, rec, voxNum, sel);
+    f3 exact = splat(0.0f);
+    if(direct_is_exact(P) && s.mats[v.me.material].specular < 1.0f)
+    {
+        // ambient + the fixed direct rays, as the lighting pass adds them
+        RayCtx c;
+        LCtx L;
+        site_ctx(P, s, c, L);
+        exact = ld3(P.ambient);
+        shadow_ray(L, v.p, 0, exact);
+    }
+    write_sample(out, v.pos, v.me, splat(0.0f), 1.0f, exact);
+}
+
+// Reference estimate of every slot's irradiance: `per` independent samples per
+// thread (per-sample random numbers), `chunks` threads per slot, summed in
+// FP64 into acc[slot] (rgb, sample count). With fixed direct rays each sample
+// is the exact ambient + direct term plus one path-traced indirect term.
+__global__ __launch_bounds__(256) void k_rc_reference(DNlightParams P, uint32_t slotsCap, uint32_t chunks, uint32_t per,
+                                                      uint32_t seed, double* acc)
+{
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t slot = g / chunks, chunk = g % chunks;
+    if(slot >= slotsCap || slot >= *P.numRequests * 32u) return;
+    const DNchunkRec* rec = nullptr;
+    uint32_t mapIndex = 0, voxNum = 0;
+    int32_t sel = -1;
+    if(!slot_voxel(P, slot, rec, mapIndex, voxNum, sel)) return;
+    const DNsceneView& s = P.scene;
+    const VoxelSite v = voxel_site(s, rec, voxNum, sel);
+    if(s.mats[v.me.material].specular >= 1.0f) return;
+    RayCtx c;
+    LCtx L;
+    site_ctx(P, s, c, L);
+    const bool exactDirect = direct_is_exact(P);
+    f3 exact = splat(0.0f);
+    if(exactDirect)
+    {
+        exact = ld3(P.ambient);
+        shadow_ray(L, v.p, 0, exact);
+    }
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for(uint32_t k = 0; k < per; k++)
+    {
+        const SampleRandom rng{pcg(seed ^ pcg(slot * 0x9E3779B9u ^ pcg(chunk * per + k)))};
+        f3 smp = splat(0.0f);
+        diffuse_ray(L, v.me.normal + DN_EPS, v.p, v.me, 0, smp, rng);
+        if(exactDirect)
+            smp = smp + exact;
+        else
+        {
+            smp = smp + ld3(P.ambient);
+            f3 dir = vnorm(L.sunDir * P.shadowSoftness + rng.ball(400u)) + DN_EPS;
+            DVoxel hv = {};
+            f3 ca; float cm;
+            f3 pp = v.p;
+            if(!light_trace(s, c, dir, vinv(dir), pp, hv, ca, cm)) smp = smp + (L.sunStrength * cm + ca);
+        }
+        sx += (double)smp.x; sy += (double)smp.y; sz += (double)smp.z;
+    }
+    double* a = acc + (size_t)slot * 4u;
+    atomicAdd(a + 0, sx);
+    atomicAdd(a + 1, sy);
+    atomicAdd(a + 2, sz);
+    atomicAdd(a + 3, (double)per);
 }
 
 __global__ __launch_bounds__(256) void k_accumulate_f64(const float* src, double* dst, size_t n)
@@ -559,6 +662,17 @@ int dn_rc_gather(const DNlightParams& p, uint32_t slots, float* samples, hipStre
 {
     if(slots == 0) return 0;
     hipLaunchKernelGGL(k_rc_gather, dim3(blocks_for(slots)), dim3(256), 0, stream, p, slots, samples);
+    return (int)hipGetLastError();
+}
+
+int dn_rc_reference(const DNlightParams& p, uint32_t slots, uint32_t samplesPerVoxel, uint32_t seed, double* acc,
+                    hipStream_t stream)
+{
+    if(slots == 0 || samplesPerVoxel == 0) return 0;
+    const uint32_t chunks = samplesPerVoxel >= 256u ? 16u : 1u;
+    const uint32_t per = (samplesPerVoxel + chunks - 1u) / chunks;
+    hipLaunchKernelGGL(k_rc_reference, dim3(blocks_for((size_t)slots * chunks)), dim3(256), 0, stream, p, slots, chunks,
+                       per, seed, acc);
     return (int)hipGetLastError();
 }
 

@@ -289,6 +289,11 @@ class RadianceCache:
         if not self.lib.DNX_rc_predict_requests(self.ptr, out.ptr, request_capacity):
             raise RuntimeError("DNX_rc_predict_requests failed")
 
+    def reference(self, acc: DeviceBuffer, request_capacity: int, samples_per_voxel: int, seed: int = 7) -> None:
+        """acc (4 doubles per slot: rgb sums, count) += independent samples of every voxel."""
+        if not self.lib.DNX_rc_reference(self.ptr, acc.ptr, request_capacity, samples_per_voxel, seed):
+            raise RuntimeError("DNX_rc_reference failed")
+
     def train_samples(self, samples: DeviceBuffer, n: int, step: bool) -> None:
         if not self.lib.DNX_rc_train_samples(self.ptr, samples.ptr, n, step):
             raise RuntimeError("DNX_rc_train_samples failed")

@@ -373,9 +373,13 @@ double DNX_batch_timing(DNXbatch* batch, uint32_t* launches, bool reset);
  * bf16 operands with fp32 accumulation on the matrix cores. It is trained
  * online with Adam on path-traced per-voxel diffuse samples drawn with
  * per-sample random numbers, and predicts each voxel's diffuse irradiance
- * (the quantity DN_update_lighting accumulates). The plain accumulator keeps
- * running beside it. A sample row is 16 floats: position in voxel units (3),
- * normal (3), albedo (3), target irradiance (3), valid 0/1, 3 unused. */
+ * (the quantity DN_update_lighting accumulates). With fixed direct rays
+ * (numDirectRays > 0) the ambient and direct terms are deterministic: they are
+ * evaluated exactly at prediction time and the network learns only the
+ * path-traced indirect term. The plain accumulator keeps running beside it.
+ * A sample row is 16 floats: position in voxel units (3), normal (3), albedo
+ * (3), target (3), valid 0/1, and an exactly known term (3) that predictions
+ * add to the network's output. */
 typedef struct DNXradianceCache DNXradianceCache;
 typedef struct DNXrcConfig {
     uint32_t log2Table;        /* 19 */
@@ -384,6 +388,8 @@ typedef struct DNXrcConfig {
     float lr, beta1, beta2, eps;   /* Adam: 1e-2, 0.9, 0.99, 1e-10 */
     uint32_t samplesPerFrame;  /* 65536 (multiple of 16) */
     uint32_t seed;             /* weight initialisation and sample draws */
+    float emaDecay;            /* predictions use a debiased moving average of the parameters over the
+                                  Adam steps with this decay (0.95); 0: the latest parameters */
 } DNXrcConfig;
 void DNX_rc_default_config(DNXrcConfig* config);
 DNXradianceCache* DNX_rc_create(DNvolume* vol, const DNXrcConfig* config /* NULL: defaults */);
@@ -403,6 +409,12 @@ bool DNX_rc_predict_samples(DNXradianceCache* rc, const float* samples, uint32_t
  * set_params also clears the gradients and the Adam state. */
 bool DNX_rc_set_params(DNXradianceCache* rc, const float* weights, const float* table);
 bool DNX_rc_get_params(DNXradianceCache* rc, float* weights, float* table, float* weightGrad, float* tableGrad);
+/* Reference estimates for scoring: adds samplesPerVoxel independent samples
+ * (per-sample random numbers; exact ambient + direct with fixed direct rays) of
+ * every stored voxel of the current request list into acc, a device array of 4
+ * doubles per slot (rgb sums, sample count), in FP64. */
+bool DNX_rc_reference(DNXradianceCache* rc, double* acc, size_t requestCapacity, uint32_t samplesPerVoxel,
+                      uint32_t seed);
 /* Device rows of the samples of the last DNX_rc_train_frame, and their count. */
 const float* DNX_rc_last_samples(DNXradianceCache* rc, uint32_t* n);
 /* dst[i] += (double)src[i] on the volume's stream (device arrays): FP64

@@ -111,13 +111,22 @@ def test_diffuse_within_mc_band(gpu):
         vol.lib.DNX_set_lighting_sample_output(vol.ptr, None, 0)
     valid = voxof != 0xFFFFFFFF
     gm, om = gs[:, valid].mean(axis=0), os_[:, valid].mean(axis=0)
-    sigma = os_[:, valid].std(axis=0, ddof=1) / np.sqrt(N)
-    band = 3.0 * np.maximum(sigma, 1e-6)
+    # Paired: both sides draw the same random directions, so frame by frame the
+    # samples differ only where FP32 and FP64 take different DDA decisions on a
+    # grazing ray (then that frame's path differs entirely). The GPU estimate is
+    # unbiased against FP64 when the mean difference stays within 3 standard
+    # errors of the per-frame differences (plus FP32 rounding, 1e-4 relative).
+    d = gs[:, valid] - os_[:, valid]
+    se = d.std(axis=0, ddof=1) / np.sqrt(N)
+    band = 3.0 * se + REL_TOL * np.maximum(np.abs(om), FLOOR)
     inside = np.abs(gm - om) <= band
     frac = float(inside.all(axis=1).mean())
+    diverged = float((np.abs(d) > REL_TOL * np.maximum(np.abs(os_[:, valid]), FLOOR)).any(axis=2).mean())
     rel = np.abs(gm - om) / np.maximum(np.abs(om), FLOOR)
-    print(f"[fp64] diffuse over {N} frames: {valid.sum()} voxels, mean |gpu - fp64| {np.abs(gm - om).mean():.2e}, "
-          f"median rel {np.median(rel):.2e}, max rel {rel.max():.2e}, mean 3-sigma band {band.mean():.2e}, "
-          f"inside band {frac:.5f}", flush=True)
+    sig = os_[:, valid].std(axis=0, ddof=1) / np.sqrt(N)
+    print(f"[fp64] diffuse over {N} frames: {valid.sum()} voxels; voxel-frames whose path differs from FP64 "
+          f"{diverged:.4f}; N-frame mean vs FP64: mean abs err {np.abs(gm - om).mean():.2e}, median rel "
+          f"{np.median(rel):.2e}, max rel {rel.max():.2e}; MC standard error of the estimate (mean) {sig.mean():.2e}; "
+          f"within 3 paired standard errors: {frac:.5f}", flush=True)
     assert frac >= 0.997
     vol.close()
