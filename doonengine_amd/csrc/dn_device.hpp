@@ -169,10 +169,18 @@ struct RayCtx {
     uint32_t hitIndex;
     uint32_t sMap, sVox;          // trips of the last step_map
     uint32_t cRays, cMap, cVox, cHits;
+    uint32_t* maskRow;            // this lane's LDS row for the chunk mask (LDS-mask walks)
 };
+
+// LDS rows for the chunk masks of the lanes of a block: 17 words per lane (16
+// mask words + 1 of padding), so the same word index of 64 lanes falls in 64
+// different banks and a voxel trip reads its mask word conflict-free.
+#define DN_MASK_ROW 17u
+#define DN_MASK_LDS_WORDS(threads) ((threads) * DN_MASK_ROW)
 
 __device__ __forceinline__ void ctx_init(RayCtx& c)
 {
+    c.maskRow = nullptr;
     c.lastVoxID = 255u; c.lastVoxRefract = 1.0f; c.hitIndex = 0xFFFFFFFFu;
     c.sMap = c.sVox = 0; c.cRays = c.cMap = c.cVox = c.cHits = 0;
 }
@@ -182,9 +190,26 @@ __device__ __forceinline__ bool occ_bit(const uint32_t* bits, uint32_t idx)
     return (bits[idx >> 5] >> (idx & 31u)) & 1u;
 }
 
+// rank_in over a mask held in an LDS row (same arithmetic)
+__device__ __forceinline__ uint32_t rank_row(const uint32_t* row, const uint32_t partial[3], uint32_t li)
+{
+    uint32_t w = li >> 5, q = w >> 2, j = w & 3u;
+    uint32_t n = q == 0 ? 0u : (q == 1 ? partial[0] : (q == 2 ? partial[1] : partial[2]));
+    const uint32_t* v = row + 4u * q;
+    uint32_t part = (1u << (li & 31u)) - 1u;
+    uint32_t b0 = j == 0 ? (v[0] & part) : v[0];
+    uint32_t b1 = j == 1 ? (v[1] & part) : (j > 1 ? v[1] : 0u);
+    uint32_t b2 = j == 2 ? (v[2] & part) : (j > 2 ? v[2] : 0u);
+    uint32_t b3 = j == 3 ? (v[3] & part) : 0u;
+    return n + __builtin_popcount(b0) + __builtin_popcount(b1) + __builtin_popcount(b2) + __builtin_popcount(b3);
+}
+
 // step_chunk, voxelShared.comp:328-418. maxDepth is negative on every path of
 // this build (no raster compositing), so the depth test at :363 always passes.
-template <bool REFRACT>
+// LDSM: the chunk's occupancy mask goes to the lane's LDS row (c.maskRow) on
+// entry and each voxel trip reads one word of it, instead of selecting the
+// word out of 16 registers by a per-lane index.
+template <bool REFRACT, bool LDSM = false>
 __device__ bool step_chunk(const DNsceneView& s, RayCtx& c, uint32_t mapIndex, f3& dir, f3& inv, f3& p,
                            bool ignoreFirst, f3& hitNormal, DVoxel& vox, f3& colorAdd, float& colorMult,
                            bool& refracted)
@@ -193,7 +218,19 @@ __device__ bool step_chunk(const DNsceneView& s, RayCtx& c, uint32_t mapIndex, f
     const DNchunkRec* rec = s.chunks + mapIndex;
     const uint4* rq = reinterpret_cast<const uint4*>(rec);
     Mask512 m;
-    m.q[0] = rq[0]; m.q[1] = rq[1]; m.q[2] = rq[2]; m.q[3] = rq[3];
+    if(LDSM)
+    {
+        uint32_t* row = c.maskRow;
+        for(int k = 0; k < 4; k++)
+        {
+            uint4 v = rq[k];
+            row[4 * k + 0] = v.x; row[4 * k + 1] = v.y; row[4 * k + 2] = v.z; row[4 * k + 3] = v.w;
+        }
+    }
+    else
+    {
+        m.q[0] = rq[0]; m.q[1] = rq[1]; m.q[2] = rq[2]; m.q[3] = rq[3];
+    }
     DDA d;
     init_dda(dir, inv, p, d);
     f3 lsd = splat(0.0f);
@@ -201,11 +238,12 @@ __device__ bool step_chunk(const DNsceneView& s, RayCtx& c, uint32_t mapIndex, f
     {
         c.sVox++;
         uint32_t li = (uint32_t)(d.px + 8 * (d.py + 8 * d.pz));
-        if(mask_bit(m, li) && !ignoreFirst)
+        const uint32_t bit = LDSM ? (c.maskRow[li >> 5] >> (li & 31u)) & 1u : mask_bit(m, li);
+        if(bit && !ignoreFirst)
         {
             uint4 tail = rq[4];   // partial[0..2], voxelIndex
             uint32_t partial[3] = {tail.x, tail.y, tail.z};
-            uint32_t idx = tail.w + rank_in(m, partial, li);
+            uint32_t idx = tail.w + (LDSM ? rank_row(c.maskRow, partial, li) : rank_in(m, partial, li));
             uint4 cw = s.voxels[idx];
             vox = decompress(cw);
             const DNmat& mat = s.mats[vox.material];
@@ -262,7 +300,7 @@ __device__ bool step_chunk(const DNsceneView& s, RayCtx& c, uint32_t mapIndex, f
 }
 
 // step_map, voxelShared.comp:421-475
-template <bool REFRACT>
+template <bool REFRACT, bool LDSM = false>
 __device__ bool step_map(const DNsceneView& s, RayCtx& c, f3& dir, f3 inv, f3& p, bool ignoreFirst, f3& hitNormal,
                          DVoxel& vox, f3& colorAdd, float& colorMult)
 {
@@ -281,12 +319,13 @@ __device__ bool step_map(const DNsceneView& s, RayCtx& c, f3& dir, f3 inv, f3& p
         uint32_t idx = (uint32_t)d.px + s.mapX * ((uint32_t)d.py + s.mapY * (uint32_t)d.pz);
         if(occ_bit(s.occLoaded, idx))
         {
-            if(s.lastUsed[idx] != 0u) s.lastUsed[idx] = 0u;
+            s.lastUsed[idx] = 0u;          // a plain store: no read on the ray's path
             f3 upd = p + dir * (minc(lsd) - DN_EPS);
             f3 crp = (upd - mk((float)d.px, (float)d.py, (float)d.pz)) * 8.0f;
             crp = vclamp(crp, DN_EPS, 8.0f - DN_EPS);
             bool refr;
-            if(step_chunk<REFRACT>(s, c, idx, dir, inv, crp, ignoreFirst, hitNormal, vox, colorAdd, colorMult, refr))
+            if(step_chunk<REFRACT, LDSM>(s, c, idx, dir, inv, crp, ignoreFirst, hitNormal, vox, colorAdd, colorMult,
+                                         refr))
             {
                 p = mk((float)d.px, (float)d.py, (float)d.pz) + crp * 0.125f;
                 hit = true;

@@ -6,18 +6,20 @@
 namespace {
 
 // One lighting ray: step_map(dir, inv = 1/dir, pos, ignoreFirst = true, ...) as
-// voxelLighting.comp calls it. The nested chunk-in-map loops are measured
-// faster here than the single flattened loop (DN_LIGHT_FLAT); both give the
-// same results bit for bit.
+// voxelLighting.comp calls it, with the chunk masks in the lane's LDS row
+// (c.maskRow, set by every kernel that traces lighting rays). The nested
+// chunk-in-map loops are measured faster here than the single flattened loop.
 __device__ __forceinline__ bool light_trace(const DNsceneView& s, RayCtx& c, f3 dir, f3 inv, f3& p, DVoxel& vox,
                                             f3& ca, float& cm)
 {
     f3 tn = splat(0.0f);
-#ifdef DN_LIGHT_FLAT
-    return step_map_flat<false>(s, c, dir, inv, p, true, vox, ca, cm, tn);
-#else
-    return step_map<false>(s, c, dir, inv, p, true, tn, vox, ca, cm);
-#endif
+    return step_map<false, true>(s, c, dir, inv, p, true, tn, vox, ca, cm);
+}
+
+// The calling lane's row of a block's mask rows.
+__device__ __forceinline__ uint32_t* lane_mask_row(uint32_t* rows)
+{
+    return rows + threadIdx.x * DN_MASK_ROW;
 }
 
 // Per-voxel context. `s` is the scene view every ray walks: the caller keeps it

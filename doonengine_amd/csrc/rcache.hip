@@ -463,9 +463,11 @@ __device__ __forceinline__ VoxelSite voxel_site(const DNsceneView& s, const DNch
     return v;
 }
 
-__device__ __forceinline__ void site_ctx(const DNlightParams& P, const DNsceneView& s, RayCtx& c, LCtx& L)
+__device__ __forceinline__ void site_ctx(const DNlightParams& P, const DNsceneView& s, RayCtx& c, LCtx& L,
+                                         uint32_t* maskRows)
 {
     ctx_init(c);
+    c.maskRow = lane_mask_row(maskRows);
     L.P = &P; L.s = &s; L.c = &c; L.dry = true; L.firstSample = false;
     L.sunDir = ld3(P.sunDir);
     L.sunStrength = ld3(s.sunStrength);
@@ -473,6 +475,7 @@ __device__ __forceinline__ void site_ctx(const DNlightParams& P, const DNsceneVi
 
 __global__ __launch_bounds__(256) void k_rc_samples(DNlightParams P, uint32_t n, uint32_t seed, float* samples)
 {
+    __shared__ uint32_t maskRows[DN_MASK_LDS_WORDS(256)];
     const uint32_t g = blockIdx.x * 256u + threadIdx.x;
     if(g >= n) return;
     const uint32_t slots = *P.numRequests * 32u;
@@ -494,7 +497,7 @@ __global__ __launch_bounds__(256) void k_rc_samples(DNlightParams P, uint32_t n,
     const DNmat& mat = s.mats[v.me.material];
     RayCtx c;
     LCtx L;
-    site_ctx(P, s, c, L);
+    site_ctx(P, s, c, L, maskRows);
     f3 diff = splat(0.0f);
     if(mat.specular < 1.0f)
     {
@@ -516,6 +519,7 @@ __global__ __launch_bounds__(256) void k_rc_samples(DNlightParams P, uint32_t n,
 // Query rows for every slot of the request list (valid 0 for empty lanes).
 __global__ __launch_bounds__(256) void k_rc_gather(DNlightParams P, uint32_t slotsCap, float* samples)
 {
+    __shared__ uint32_t maskRows[DN_MASK_LDS_WORDS(256)];
     const uint32_t g = blockIdx.x * 256u + threadIdx.x;
     if(g >= slotsCap) return;
     float* out = samples + (size_t)g * RC_SAMPLE;
@@ -535,7 +539,7 @@ __global__ __launch_bounds__(256) void k_rc_gather(DNlightParams P, uint32_t slo
         // ambient + the fixed direct rays, as the lighting pass adds them
         RayCtx c;
         LCtx L;
-        site_ctx(P, s, c, L);
+        site_ctx(P, s, c, L, maskRows);
         exact = ld3(P.ambient);
         shadow_ray(L, v.p, 0, exact);
     }
@@ -549,6 +553,7 @@ __global__ __launch_bounds__(256) void k_rc_gather(DNlightParams P, uint32_t slo
 __global__ __launch_bounds__(256) void k_rc_reference(DNlightParams P, uint32_t slotsCap, uint32_t chunks, uint32_t per,
                                                       uint32_t seed, double* acc)
 {
+    __shared__ uint32_t maskRows[DN_MASK_LDS_WORDS(256)];
     const uint32_t g = blockIdx.x * 256u + threadIdx.x;
     const uint32_t slot = g / chunks, chunk = g % chunks;
     if(slot >= slotsCap || slot >= *P.numRequests * 32u) return;
@@ -561,7 +566,7 @@ __global__ __launch_bounds__(256) void k_rc_reference(DNlightParams P, uint32_t 
     if(s.mats[v.me.material].specular >= 1.0f) return;
     RayCtx c;
     LCtx L;
-    site_ctx(P, s, c, L);
+    site_ctx(P, s, c, L, maskRows);
     const bool exactDirect = direct_is_exact(P);
     f3 exact = splat(0.0f);
     if(exactDirect)
