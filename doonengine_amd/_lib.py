@@ -157,6 +157,7 @@ SIGNATURES = {
     "DNX_set_message_callback": (None, [MSG_CB]),
     "DNX_error_count": (c_uint32, []),
     "DNX_set_voxels_dense": (None, [POINTER(DNvolume), DNivec3, DNuvec3, c_void_p, c_void_p]),
+    "DNX_get_voxels_dense": (None, [POINTER(DNvolume), DNivec3, DNuvec3, c_void_p, c_void_p]),
     "DNX_request_all_chunks": (None, [POINTER(DNvolume)]),
     "DNX_mark_visible": (c_uint32, [POINTER(DNvolume), c_uint32]),
     "DNX_get_stream": (c_void_p, [POINTER(DNvolume)]),

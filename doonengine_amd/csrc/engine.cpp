@@ -134,7 +134,7 @@ struct DNgpuVolume {
     uint32_t* dRequests = nullptr;
     uint32_t* dNumRequests = nullptr;
     uint32_t* dBlockSums = nullptr;
-    uint32_t* dWorkCounter = nullptr;
+    uint32_t* dWorkCounter = nullptr;  // [0] lighting requests taken, [1] collected requests, [2..3] draw quads
     uint32_t* dReqList = nullptr;      // tiles requested by rays (collect)
     uint64_t* dCounters = nullptr;     // 3 kinds x CNT_SLOTS
     float* dPoints = nullptr;          // 2 x DN_MAX_POINTS x 3
@@ -1140,6 +1140,8 @@ extern "C" void DN_draw(DNvolume* vol, DNtexture out, DNmat4 view, DNmat4 projec
     if(g->drawHitOut && !p.hitOut)
         dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR, "draw hit output smaller than the image; not written");
     p.counters = g->dCounters + 0 * CNT_SLOTS;
+    p.quadCounter = g->dWorkCounter + 2;
+    p.tiles = g->tiles;
     Timer tm = timer_begin(g, TK_DRAW);
     int e = dn_launch_draw(p, g->stream);
     timer_end(g, tm);
@@ -1458,10 +1460,13 @@ extern "C" DNvec3 DN_cam_dir(DNvec3 orient)
 
 extern "C" DNcompressedVoxel DN_compress_voxel(DNvoxel voxel)
 {
+    // min(x, 1) = x < 1 ? x : 1 and max(x, -1) = x > -1 ? x : -1, so a NaN
+    // component (fromCenter / 0 at a sphere's centre voxel) quantises as 1.
     DNvec3 n = voxel.normal;
-    n.x = std::max(std::min(n.x, 1.0f), -1.0f);
-    n.y = std::max(std::min(n.y, 1.0f), -1.0f);
-    n.z = std::max(std::min(n.z, 1.0f), -1.0f);
+    auto clamp1 = [](float x) { x = x < 1.0f ? x : 1.0f; return x > -1.0f ? x : -1.0f; };
+    n.x = clamp1(n.x);
+    n.y = clamp1(n.y);
+    n.z = clamp1(n.z);
     uint32_t nx = (uint32_t)(((int)(n.x * 255.0f) + 255) / 2), ny = (uint32_t)(((int)(n.y * 255.0f) + 255) / 2),
              nz = (uint32_t)(((int)(n.z * 255.0f) + 255) / 2);
     DNcompressedVoxel r;
@@ -1502,13 +1507,31 @@ extern "C" void DN_shape_sphere(DNvolume* vol, DNvoxel voxel, bool flipNormals, 
     mapMax.y = std::min(mapMax.y, (int)vol->mapSize.y - 1);
     mapMax.z = std::min(mapMax.z, (int)vol->mapSize.z - 1);
     const float r2 = r * r, r12 = (r + 1.0f) * (r + 1.0f);
+    const bool shell = voxel.material == DN_MATERIAL_EMPTY && flipNormals;
+    // A voxel is touched only when its FP32 squared distance is below r2 (or
+    // r12 for the normal-flipping shell). Rounding is monotone, so the voxel of
+    // a tile (or of a row) nearest to the centre along each axis has the
+    // smallest FP32 distance of all of them: tiles and rows whose nearest voxel
+    // fails both tests are skipped without changing which voxels are written
+    // or in what order.
+    auto touched = [&](float d2) { return d2 < r2 || (shell && d2 < r12); };
+    auto nearest = [](int base, float cc) {
+        float m = fabsf((float)base - cc);
+        for(int k = 1; k < 8; k++) m = std::min(m, fabsf((float)(base + k) - cc));
+        return m;
+    };
     for(int mZ = mapMin.z; mZ <= mapMax.z; mZ++)
     for(int mY = mapMin.y; mY <= mapMax.y; mY++)
     for(int mX = mapMin.x; mX <= mapMax.x; mX++)
     {
         DNivec3 mp = {mX, mY, mZ};
+        const float nx = nearest(mX * 8, c.x), ny = nearest(mY * 8, c.y), nz = nearest(mZ * 8, c.z);
+        if(!touched(nx * nx + ny * ny + nz * nz)) continue;
         for(int cZ = 0; cZ < 8; cZ++)
         for(int cY = 0; cY < 8; cY++)
+        {
+            const float fy = (float)(mY * 8 + cY) - c.y, fz = (float)(mZ * 8 + cZ) - c.z;
+            if(!touched(nx * nx + fy * fy + fz * fz)) continue;
         for(int cX = 0; cX < 8; cX++)
         {
             DNivec3 cp = {cX, cY, cZ};
@@ -1537,6 +1560,7 @@ extern "C" void DN_shape_sphere(DNvolume* vol, DNvoxel voxel, bool flipNormals, 
                 old.normal = v3(fc.x * s, fc.y * s, fc.z * s);
                 DN_set_voxel(vol, mp, cp, old);
             }
+        }
         }
     }
 }
@@ -1567,6 +1591,26 @@ extern "C" void DNX_set_voxels_dense(DNvolume* vol, DNivec3 origin, DNuvec3 size
         uint32_t mi = flat(vol, mp);
         if(vol->map[mi].flag == 0 && material_of(v.normal) == DN_MATERIAL_EMPTY) continue;
         DN_set_compressed_voxel(vol, mp, cp, v);
+    }
+}
+
+extern "C" void DNX_get_voxels_dense(DNvolume* vol, DNivec3 origin, DNuvec3 size, uint32_t* normalWords,
+                                     uint32_t* albedoWords)
+{
+    size_t k = 0;
+    for(uint32_t z = 0; z < size.z; z++)
+    for(uint32_t y = 0; y < size.y; y++)
+    for(uint32_t x = 0; x < size.x; x++, k++)
+    {
+        DNivec3 p = {origin.x + (int32_t)x, origin.y + (int32_t)y, origin.z + (int32_t)z};
+        normalWords[k] = UINT32_MAX;
+        albedoWords[k] = 0u;
+        if(p.x < 0 || p.y < 0 || p.z < 0) continue;
+        DNivec3 mp = {p.x / 8, p.y / 8, p.z / 8}, cp = {p.x % 8, p.y % 8, p.z % 8};
+        if(!DN_in_map_bounds(vol, mp) || vol->map[flat(vol, mp)].flag == 0) continue;
+        DNcompressedVoxel v = DN_get_compressed_voxel(vol, mp, cp);
+        normalWords[k] = v.normal;
+        albedoWords[k] = v.albedo;
     }
 }
 

@@ -29,72 +29,108 @@ __device__ f3 voxel_color(const DNsceneView& s, uint32_t viewMode, DVoxel vox, f
     }
 }
 
-__global__ __launch_bounds__(256) void k_draw(DNdrawParams P)
+// One pixel's primary ray (voxelDraw.comp:24-147). The per-ray refraction state
+// starts afresh for every pixel, as the shader's per-invocation globals do.
+__device__ __forceinline__ void draw_pixel(const DNdrawParams& P, const DNsceneView& s, RayCtx& c, uint32_t x,
+                                           uint32_t y)
 {
-    const DNsceneView& s = P.scene;
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t x = blockIdx.x * 16u + (wave & 1u) * 8u + (lane & 7u);
-    const uint32_t y = blockIdx.y * 16u + (wave >> 1) * 8u + (lane >> 3);
+    c.lastVoxID = 255u; c.lastVoxRefract = 1.0f;
+    const float* M = P.invViewProj;
+    float sx = (float)x / (float)P.width * 2.0f - 1.0f;
+    float sy = (float)y / (float)P.height * 2.0f - 1.0f;
+    f3 org = mk(P.invView[12], P.invView[13], P.invView[14]);
+    f3 p = org;
+    f3 q = mk(M[0] * sx + M[4] * sy + M[8] * 0.0f + M[12],
+              M[1] * sx + M[5] * sy + M[9] * 0.0f + M[13],
+              M[2] * sx + M[6] * sy + M[10] * 0.0f + M[14]);
+    f3 dir = vnorm(q) + DN_EPS;
+    f3 inv = vinv(dir);
+    f3 ms = mk((float)s.mapX, (float)s.mapY, (float)s.mapZ);
+    f3 color;
+    float depth = -1.0f;
+    PixelHit info = {0xFFFFFFFFu, 0u, 0u, 0u, {0.0f, 0.0f, 0.0f}};
+    f3 tMin = (splat(0.0f) - p) * inv, tMax = (ms - p) * inv;
+    f3 t1 = vmin(tMin, tMax), t2 = vmax(tMin, tMax);
+    float tNear = gmax(gmax(t1.x, t1.y), t1.z), tFar = gmin(gmin(t2.x, t2.y), t2.z);
+    if(tNear > tFar || tFar < 0.0f)
+        color = vpow(sky_color(s, dir), 2.2f);
+    else
+    {
+        if(tNear > 0.0f) p = p + dir * (tNear + DN_EPS);
+        // normal_AABB, voxelShared.comp:276-284
+        f3 cen = (splat(0.0f) + ms) * 0.5f;
+        f3 pc = p - cen;
+        f3 dd = (ms - splat(0.0f)) * 0.5f;
+        float bias = 1.0f + DN_EPS;
+        f3 tr = mk(truncf(pc.x / dd.x * bias), truncf(pc.y / dd.y * bias), truncf(pc.z / dd.z * bias));
+        f3 nrm = vnorm(tr);
+        DVoxel vox = {};
+        f3 ca; float cm;
+        bool hit = step_map<true, true>(s, c, dir, inv, p, false, nrm, vox, ca, cm);
+        info.mapSteps = c.sMap; info.voxelSteps = c.sVox;
+        if(hit)
+        {
+            f3 fp = vfloor(p);
+            if(fp.x >= 0.0f && fp.y >= 0.0f && fp.z >= 0.0f && fp.x < ms.x && fp.y < ms.y && fp.z < ms.z)
+            {
+                uint32_t idx = (uint32_t)fp.x + s.mapX * ((uint32_t)fp.y + s.mapY * (uint32_t)fp.z);
+                if(!(s.flags[idx] & 4u)) atomicOr(&s.flags[idx], 4u);
+            }
+            f3 dv = p - org;
+            depth = sqrtf(dot3(dv, dv));
+            color = voxel_color(s, P.viewMode, vox, ca, cm, nrm);
+            info.voxelIndex = c.hitIndex;
+        }
+        else
+            color = vpow(sky_color(s, dir), 2.2f) * cm + ca;
+        info.face[0] = nrm.x; info.face[1] = nrm.y; info.face[2] = nrm.z;
+    }
+    color = vpow(color, 0.4545f);
+    size_t o = (size_t)y * P.width + x;
+    P.out[o] = make_float4(color.x, color.y, color.z, depth);
+    if(P.hitOut) reinterpret_cast<PixelHit*>(P.hitOut)[o] = info;
+}
+
+// Persistent draw: each wave takes 8x8 pixel quads (numbered row by row) from
+// a device counter, lane l drawing pixel (l & 7, l >> 3) of the quad. The
+// chunk-grid occupancy bitmap is staged in LDS once per block, and chunk masks
+// go to per-lane LDS rows on chunk entry. The last wave to finish resets the
+// quad counter for the next launch.
+#ifndef DN_DRAW_WAVES
+#define DN_DRAW_WAVES 4
+#endif
+template <bool LDS_OCC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DN_DRAW_WAVES, DN_DRAW_WAVES)))
+void k_draw(DNdrawParams P)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    __shared__ uint32_t maskRows[DN_MASK_LDS_WORDS(256)];
+    extern __shared__ uint32_t occLds[];
+    DNsceneView sv = P.scene;
+    if(LDS_OCC)
+    {
+        const uint32_t words = (P.tiles + 31u) / 32u;
+        for(uint32_t i = threadIdx.x; i < words; i += 256u) occLds[i] = P.scene.occLoaded[i];
+        __syncthreads();
+        sv.occLoaded = occLds;
+    }
     RayCtx c;
     ctx_init(c);
+    c.maskRow = maskRows + threadIdx.x * DN_MASK_ROW;
+    const uint32_t qx = (P.width + 7u) / 8u, nq = qx * ((P.height + 7u) / 8u);
     uint32_t drawn = 0;
-    if(x < P.width && y < P.height)
+    for(;;)
     {
-        drawn = 1;
-        const float* M = P.invViewProj;
-        float sx = (float)x / (float)P.width * 2.0f - 1.0f;
-        float sy = (float)y / (float)P.height * 2.0f - 1.0f;
-        f3 org = mk(P.invView[12], P.invView[13], P.invView[14]);
-        f3 p = org;
-        f3 q = mk(M[0] * sx + M[4] * sy + M[8] * 0.0f + M[12],
-                  M[1] * sx + M[5] * sy + M[9] * 0.0f + M[13],
-                  M[2] * sx + M[6] * sy + M[10] * 0.0f + M[14]);
-        f3 dir = vnorm(q) + DN_EPS;
-        f3 inv = vinv(dir);
-        f3 ms = mk((float)s.mapX, (float)s.mapY, (float)s.mapZ);
-        f3 color;
-        float depth = -1.0f;
-        PixelHit info = {0xFFFFFFFFu, 0u, 0u, 0u, {0.0f, 0.0f, 0.0f}};
-        f3 tMin = (splat(0.0f) - p) * inv, tMax = (ms - p) * inv;
-        f3 t1 = vmin(tMin, tMax), t2 = vmax(tMin, tMax);
-        float tNear = gmax(gmax(t1.x, t1.y), t1.z), tFar = gmin(gmin(t2.x, t2.y), t2.z);
-        if(tNear > tFar || tFar < 0.0f)
-            color = vpow(sky_color(s, dir), 2.2f);
-        else
+        uint32_t q = 0;
+        if(lane == 0) q = atomicAdd(P.quadCounter, 1u);
+        q = __shfl(q, 0, 64);
+        if(q >= nq) break;
+        const uint32_t x = (q % qx) * 8u + (lane & 7u), y = (q / qx) * 8u + (lane >> 3);
+        if(x < P.width && y < P.height)
         {
-            if(tNear > 0.0f) p = p + dir * (tNear + DN_EPS);
-            // normal_AABB, voxelShared.comp:276-284
-            f3 cen = (splat(0.0f) + ms) * 0.5f;
-            f3 pc = p - cen;
-            f3 dd = (ms - splat(0.0f)) * 0.5f;
-            float bias = 1.0f + DN_EPS;
-            f3 tr = mk(truncf(pc.x / dd.x * bias), truncf(pc.y / dd.y * bias), truncf(pc.z / dd.z * bias));
-            f3 nrm = vnorm(tr);
-            DVoxel vox = {};
-            f3 ca; float cm;
-            bool hit = step_map<true>(s, c, dir, inv, p, false, nrm, vox, ca, cm);
-            info.mapSteps = c.sMap; info.voxelSteps = c.sVox;
-            if(hit)
-            {
-                f3 fp = vfloor(p);
-                if(fp.x >= 0.0f && fp.y >= 0.0f && fp.z >= 0.0f && fp.x < ms.x && fp.y < ms.y && fp.z < ms.z)
-                {
-                    uint32_t idx = (uint32_t)fp.x + s.mapX * ((uint32_t)fp.y + s.mapY * (uint32_t)fp.z);
-                    if(!(s.flags[idx] & 4u)) atomicOr(&s.flags[idx], 4u);
-                }
-                f3 dv = p - org;
-                depth = sqrtf(dot3(dv, dv));
-                color = voxel_color(s, P.viewMode, vox, ca, cm, nrm);
-                info.voxelIndex = c.hitIndex;
-            }
-            else
-                color = vpow(sky_color(s, dir), 2.2f) * cm + ca;
-            info.face[0] = nrm.x; info.face[1] = nrm.y; info.face[2] = nrm.z;
+            drawn++;
+            draw_pixel(P, sv, c, x, y);
         }
-        color = vpow(color, 0.4545f);
-        size_t o = (size_t)y * P.width + x;
-        P.out[o] = make_float4(color.x, color.y, color.z, depth);
-        if(P.hitOut) reinterpret_cast<PixelHit*>(P.hitOut)[o] = info;
     }
     if(P.counters)
     {
@@ -104,8 +140,18 @@ __global__ __launch_bounds__(256) void k_draw(DNdrawParams P)
         wave_add_u64(P.counters + CNT_HITS, c.cHits);
         wave_add_u64(P.counters + CNT_PIXELS, drawn);
     }
+    if(lane == 0)
+    {
+        // every wave has taken its last quad before it counts itself out
+        __threadfence();
+        const uint32_t done = atomicAdd(P.quadCounter + 1, 1u);
+        if(done == gridDim.x * 4u - 1u)
+        {
+            atomicExch(P.quadCounter, 0u);
+            atomicExch(P.quadCounter + 1, 0u);
+        }
+    }
 }
-
 
 }  // namespace
 
@@ -530,8 +576,26 @@ int cu_count()
 
 int dn_launch_draw(const DNdrawParams& p, hipStream_t stream)
 {
-    dim3 grid((p.width + 15) / 16, (p.height + 15) / 16);
-    hipLaunchKernelGGL(k_draw, grid, dim3(256), 0, stream, p);
+    // persistent grid of resident blocks, no more than one wave per quad
+    static int perCU[2] = {0, 0};
+    static size_t lastOccBytes = ~(size_t)0;
+    const size_t occBytes = sizeof(uint32_t) * ((p.tiles + 31u) / 32u);
+    const bool lds = occBytes <= DN_OCC_LDS_MAX;
+    if(perCU[lds] == 0 || (lds && occBytes != lastOccBytes))
+    {
+        int n = 0;
+        if(lds) hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_draw<true>, 256, occBytes);
+        else hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_draw<false>, 256, 0);
+        perCU[lds] = n < 1 ? 1 : n;
+        if(lds) lastOccBytes = occBytes;
+    }
+    const uint32_t quads = ((p.width + 7u) / 8u) * ((p.height + 7u) / 8u);
+    uint32_t blocks = (uint32_t)cu_count() * (uint32_t)perCU[lds];
+    if((quads + 3u) / 4u < blocks) blocks = (quads + 3u) / 4u;
+    if(lds)
+        hipLaunchKernelGGL(k_draw<true>, dim3(blocks), dim3(256), occBytes, stream, p);
+    else
+        hipLaunchKernelGGL(k_draw<false>, dim3(blocks), dim3(256), 0, stream, p);
     return (int)hipGetLastError();
 }
 

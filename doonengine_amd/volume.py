@@ -101,6 +101,17 @@ class Volume:
         z, y, x = n.shape
         self.lib.DNX_set_voxels_dense(self.ptr, DNivec3(*origin), DNuvec3(x, y, z), n.ctypes.data, a.ctypes.data)
 
+    def get_dense(self, size, origin=(0, 0, 0)):
+        """Reads a dense (Z, Y, X) box of voxel words (DNX_get_voxels_dense): (normal, albedo)."""
+        x, y, z = (int(v) for v in size)
+        n = np.empty((z, y, x), np.uint32)
+        a = np.empty((z, y, x), np.uint32)
+        self.lib.DNX_get_voxels_dense(self.ptr, DNivec3(*origin), DNuvec3(x, y, z), n.ctypes.data, a.ctypes.data)
+        return n, a
+
+    def chunk_exists(self, map_pos) -> bool:
+        return bool(self.lib.DN_does_chunk_exist(self.ptr, DNivec3(*map_pos)))
+
     def set_voxel(self, map_pos, chunk_pos, material: int, normal, albedo) -> None:
         vox = _lib.DNvoxel(material, DNvec3(*normal), _lib.DNcolor(*albedo))
         self.lib.DN_set_voxel(self.ptr, DNivec3(*map_pos), DNivec3(*chunk_pos), vox)

@@ -246,6 +246,10 @@ uint32_t DNX_error_count(void);
  * encoding; material 255 means empty. Same effect as DN_set_compressed_voxel per voxel. */
 void DNX_set_voxels_dense(DNvolume* vol, DNivec3 origin, DNuvec3 size,
                           const uint32_t* normalWords, const uint32_t* albedoWords);
+/* Reads the same dense box back (DN_get_compressed_voxel per voxel). Voxels of
+ * tiles without a chunk, or outside the map, read as the cleared word pair
+ * (normal 0xFFFFFFFF, albedo 0). */
+void DNX_get_voxels_dense(DNvolume* vol, DNivec3 origin, DNuvec3 size, uint32_t* normalWords, uint32_t* albedoWords);
 
 /* Residency: marks every chunk present on the CPU to be (re)loaded by the next
  * DN_sync_gpu, instead of waiting for rays to request it. */

@@ -43,6 +43,44 @@ class ORlightParams(Structure):
                 ("directPoints", c_void_p)]
 
 
+class ORgrid(Structure):
+    _fields_ = [("mx", ctypes.c_int32), ("my", ctypes.c_int32), ("mz", ctypes.c_int32), ("normal", c_void_p),
+                ("albedo", c_void_p), ("flag", c_void_p), ("count", c_void_p)]
+
+
+class EditGrid:
+    """Dense host-edit model (voxel.c:1126-1160, voxelShapes.c:239-333): words
+    (Z, Y, X) per voxel, chunk-present flag and stored-voxel count per tile."""
+
+    def __init__(self, oracle: "Oracle", map_size):
+        mx, my, mz = (int(v) for v in map_size)
+        self.o = oracle
+        self.normal = np.full((8 * mz, 8 * my, 8 * mx), 0xFFFFFFFF, np.uint32)
+        self.albedo = np.zeros((8 * mz, 8 * my, 8 * mx), np.uint32)
+        self.flag = np.zeros((mz, my, mx), np.uint8)
+        self.count = np.zeros((mz, my, mx), np.int32)
+        self.g = ORgrid(mx, my, mz, self.normal.ctypes.data, self.albedo.ctypes.data, self.flag.ctypes.data,
+                        self.count.ctypes.data)
+
+    def set_word(self, x: int, y: int, z: int, normal: int, albedo: int) -> None:
+        self.o.lib.or_set_voxel_word(ctypes.byref(self.g), int(x), int(y), int(z), c_uint32(normal), c_uint32(albedo))
+
+    def set_dense(self, normal_words: np.ndarray, albedo_words: np.ndarray) -> None:
+        """DNX_set_voxels_dense from the origin: voxel by voxel, x fastest."""
+        zs, ys, xs = normal_words.shape
+        for z in range(zs):
+            for y in range(ys):
+                for x in range(xs):
+                    self.set_word(x, y, z, int(normal_words[z, y, x]), int(albedo_words[z, y, x]))
+
+    def shape_sphere(self, material: int, albedo, center, radius: float, flip_normals=False, normal=(0, 0, 0)):
+        n = np.asarray(normal, np.float32)
+        word = (int(albedo[0]) << 24) | (int(albedo[1]) << 16) | (int(albedo[2]) << 8)
+        self.o.lib.or_shape_sphere(ctypes.byref(self.g), c_uint32(material), n.ctypes.data_as(c_void_p),
+                                   c_uint32(word), c_int(bool(flip_normals)), c_float(center[0]),
+                                   c_float(center[1]), c_float(center[2]), c_float(radius))
+
+
 RAYHIT_DTYPE = np.dtype([("hit", "<u4"), ("voxelIndex", "<u4"), ("mapSteps", "<u4"), ("voxelSteps", "<u4"),
                          ("pos", "<f4", 3), ("normal", "<f4", 3)])
 PIXEL_DTYPE = np.dtype([("voxelIndex", "<u4"), ("mapSteps", "<u4"), ("voxelSteps", "<u4"), ("pad", "<u4"),
@@ -125,6 +163,9 @@ class Oracle:
         out = np.zeros(3, np.float32)
         self.lib.or_rand_unit_sphere(c_float(seed), out.ctypes.data_as(c_void_p))
         return out
+
+    def edit_grid(self, map_size) -> EditGrid:
+        return EditGrid(self, map_size)
 
     def chunk_to_gpu(self, mats: np.ndarray, cpu_voxels: np.ndarray):
         """cpu_voxels: (8, 8, 8, 2) u32 indexed [x][y][z] -> (chunk (24,), voxels (n, 4))."""

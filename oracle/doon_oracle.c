@@ -961,3 +961,138 @@ void or_decompress(const uint32_t* w, real* out)
     out[9] = v.diffuse.x; out[10] = v.diffuse.y; out[11] = v.diffuse.z;
     out[12] = (real)v.material;
 }
+
+/* ---------------------------------------------------------------------------
+ * Host voxel edits on a dense grid (always FP32, as the reference's C code).
+ * The grid holds X*Y*Z voxels (X = 8 * map width ...), voxel (x, y, z) at
+ * x + X*(y + Y*z); per tile a chunk-present flag and its stored-voxel count.
+ * A tile without a chunk holds the cleared words (normal 0xFFFFFFFF, albedo 0),
+ * the state _DN_clear_chunk leaves (voxel.c:1353-1363), which is also what a
+ * newly added chunk starts from. Chunk pool slots are not modelled.
+ */
+typedef struct {
+    int32_t mx, my, mz;        /* map size in tiles */
+    uint32_t* normal;          /* per voxel */
+    uint32_t* albedo;          /* per voxel */
+    uint8_t* flag;             /* per tile: chunk present */
+    int32_t* count;            /* per tile: stored voxels */
+} ORgrid;
+
+static inline uint32_t g_vox(const ORgrid* g, int x, int y, int z)
+{
+    return (uint32_t)x + (uint32_t)(8 * g->mx) * ((uint32_t)y + (uint32_t)(8 * g->my) * (uint32_t)z);
+}
+
+static void g_clear_tile(ORgrid* g, int tx, int ty, int tz)
+{
+    for(int z = 0; z < 8; z++)
+    for(int y = 0; y < 8; y++)
+    for(int x = 0; x < 8; x++)
+    {
+        uint32_t v = g_vox(g, tx * 8 + x, ty * 8 + y, tz * 8 + z);
+        g->normal[v] = 0xFFFFFFFFu;
+        g->albedo[v] = 0u;
+    }
+}
+
+/* DN_set_compressed_voxel, voxel.c:1126-1160: an empty voxel into a missing
+ * chunk does nothing, a solid one adds the chunk; the count follows empty<->solid
+ * changes and the chunk is removed (cleared) when it reaches zero. */
+void or_set_voxel_word(ORgrid* g, int x, int y, int z, uint32_t normal, uint32_t albedo)
+{
+    int tx = x / 8, ty = y / 8, tz = z / 8;
+    uint32_t t = (uint32_t)tx + (uint32_t)g->mx * ((uint32_t)ty + (uint32_t)g->my * (uint32_t)tz);
+    uint32_t v = g_vox(g, x, y, z);
+    if(!g->flag[t])
+    {
+        if((normal >> 24) == 255u) return;
+        g->flag[t] = 1;
+    }
+    uint32_t oldMat = g->normal[v] >> 24, newMat = normal >> 24;
+    if(oldMat == 255u && newMat != 255u) g->count[t]++;
+    else if(oldMat != 255u && newMat == 255u)
+    {
+        if(--g->count[t] <= 0)
+        {
+            g->flag[t] = 0;
+            g->count[t] = 0;
+            g_clear_tile(g, tx, ty, tz);
+            return;
+        }
+    }
+    g->normal[v] = normal;
+    g->albedo[v] = albedo;
+}
+
+/* DN_compress_voxel, voxel.c:1289-1301 */
+static uint32_t g_compress_normal(uint32_t material, float nx, float ny, float nz)
+{
+    nx = nx < 1.0f ? nx : 1.0f; ny = ny < 1.0f ? ny : 1.0f; nz = nz < 1.0f ? nz : 1.0f;
+    nx = nx > -1.0f ? nx : -1.0f; ny = ny > -1.0f ? ny : -1.0f; nz = nz > -1.0f ? nz : -1.0f;
+    uint32_t qx = (uint32_t)(((int)(nx * 255.0f) + 255) / 2), qy = (uint32_t)(((int)(ny * 255.0f) + 255) / 2),
+             qz = (uint32_t)(((int)(nz * 255.0f) + 255) / 2);
+    return (material << 24) | (qx << 16) | (qy << 8) | qz;
+}
+
+/* DN_shape_sphere without the transform callback, voxelShapes.c:239-333: every
+ * voxel of the (clamped) tile box, tile by tile in z, y, x order and voxel by
+ * voxel in z, y, x order inside a tile; FP32 squared distance from the centre,
+ * dot = (x*x + y*y) + z*z. Solid spheres skip voxels that exist and write the
+ * normal fromCenter / max|fromCenter|; empty ones write the caller's (empty)
+ * voxel over every voxel inside and,
+ * with flipNormals, turn the normals of the voxels in the next shell inwards.
+ * The chunk-present flag is read once per tile, as the reference does. */
+void or_shape_sphere(ORgrid* g, uint32_t material, const float* normal, uint32_t albedoWord, int flipNormals,
+                     float cx, float cy, float cz, float r)
+{
+    int iMin[3] = {(int)floorf(cx - r), (int)floorf(cy - r), (int)floorf(cz - r)};
+    int iMax[3] = {(int)ceilf(cx + r), (int)ceilf(cy + r), (int)ceilf(cz + r)};
+    if(material == 255u)
+        for(int a = 0; a < 3; a++) { iMin[a]--; iMax[a]++; }
+    int size[3] = {g->mx, g->my, g->mz}, lo[3], hi[3];
+    for(int a = 0; a < 3; a++)
+    {
+        lo[a] = iMin[a] / 8;              /* C division: truncation, as DN_separate_position */
+        hi[a] = iMax[a] / 8;
+        if(lo[a] < 0) lo[a] = 0;
+        if(hi[a] > size[a] - 1) hi[a] = size[a] - 1;
+    }
+    const float r2 = r * r, r12 = (r + 1.0f) * (r + 1.0f);
+    for(int mZ = lo[2]; mZ <= hi[2]; mZ++)
+    for(int mY = lo[1]; mY <= hi[1]; mY++)
+    for(int mX = lo[0]; mX <= hi[0]; mX++)
+    {
+        uint32_t t = (uint32_t)mX + (uint32_t)g->mx * ((uint32_t)mY + (uint32_t)g->my * (uint32_t)mZ);
+        int tileFlag = g->flag[t];
+        for(int cZ = 0; cZ < 8; cZ++)
+        for(int cY = 0; cY < 8; cY++)
+        for(int cX = 0; cX < 8; cX++)
+        {
+            int x = mX * 8 + cX, y = mY * 8 + cY, z = mZ * 8 + cZ;
+            float fx = (float)x - cx, fy = (float)y - cy, fz = (float)z - cz;
+            float d2 = fx * fx + fy * fy + fz * fz;
+            uint32_t v = g_vox(g, x, y, z);
+            if(d2 < r2)
+            {
+                if(tileFlag && (g->normal[v] >> 24) != 255u && material != 255u) continue;
+                float nx = normal[0], ny = normal[1], nz = normal[2];
+                if(material != 255u)
+                {
+                    float m = fmaxf(fmaxf(fabsf(fx), fabsf(fy)), fabsf(fz));
+                    float s = 1.0f / m;
+                    nx = fx * s; ny = fy * s; nz = fz * s;
+                }
+                or_set_voxel_word(g, x, y, z, g_compress_normal(material, nx, ny, nz), albedoWord);
+            }
+            else if(material == 255u && flipNormals && d2 < r12)
+            {
+                if(!tileFlag || (g->normal[v] >> 24) == 255u) continue;
+                float m = fmaxf(fmaxf(fabsf(fx), fabsf(fy)), fabsf(fz));
+                float s = -1.0f / m;
+                /* DN_get_voxel then DN_set_voxel: the albedo keeps its three colour bytes */
+                or_set_voxel_word(g, x, y, z, g_compress_normal(g->normal[v] >> 24, fx * s, fy * s, fz * s),
+                                  g->albedo[v] & 0xFFFFFF00u);
+            }
+        }
+    }
+}
