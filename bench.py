@@ -232,6 +232,18 @@ def bench_edits(args) -> int:
     kk, jj = np.meshgrid(np.arange(256), np.arange(6), indexing="ij")
     edit_u = [uniform01(kk, jj, seed=4 + f).tolist()
               for f in range(frames + args.warmup)]
+    frame_edits = []
+    for us in edit_u:
+        fe = []
+        for u in us:
+            r = 3.0 + 21.0 * u[0]
+            ctr = (u[1] * X, u[2] * Y, u[3] * Z)
+            if u[4] < 0.5:
+                fe.append((255, (0, 0, 0), ctr, r, False))
+            else:
+                m = min(n_mats - 1, int(u[5] * n_mats))
+                fe.append((m, sc.materials[m].albedo, ctr, r, False))
+        frame_edits.append(Volume.sphere_edits(fe) if args.edit_api == "batch" else fe)
     times, edit_ms, up_bytes, up_chunks, lit = [], [], [], [], []
     vol.counters(KIND_LIGHTING, reset=True)
     vol.lib.DNX_get_upload_stats(vol.ptr, None, None, None, True)
@@ -242,14 +254,11 @@ def bench_edits(args) -> int:
         yaw = float(np.degrees(np.arctan2(tan[0], tan[2])))
         vol.set_vec("camOrient", (25.0, yaw, 0.0))
         t0 = time.perf_counter()
-        for u in edit_u[f]:
-            r = 3.0 + 21.0 * u[0]
-            ctr = (u[1] * X, u[2] * Y, u[3] * Z)
-            if u[4] < 0.5:
-                vol.shape_sphere(255, (0, 0, 0), ctr, r)
-            else:
-                m = min(n_mats - 1, int(u[5] * n_mats))
-                vol.shape_sphere(m, sc.materials[m].albedo, ctr, r)
+        if args.edit_api == "batch":
+            vol.shape_spheres(frame_edits[f])
+        else:
+            for mat, alb, ctr, r, flip in frame_edits[f]:
+                vol.shape_sphere(mat, alb, ctr, r, flip_normals=flip)
         t1 = time.perf_counter()
         view, proj = vol.view_projection(H / W)
         vol.draw(out, W, H, view, proj)
@@ -275,7 +284,9 @@ def bench_edits(args) -> int:
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "16x16x16 chunks (128^3 voxels) fBm ~35%, 256 sphere edits/frame (radius 3-24, "
                                "carve/fill 50/50), 256x256 draw, 16 direct/specular rays, 1 diffuse sample",
-                   "frames": frames, "parallelism": "single GPU"},
+                   "frames": frames, "edit_api": "DNX_shape_spheres (device, one call per frame)"
+                   if args.edit_api == "batch" else "DN_shape_sphere (host, one call per edit)",
+                   "parallelism": "single GPU"},
         "frame_ms_p50": float(1e3 * np.percentile(t, 50)), "frame_ms_p99": float(1e3 * np.percentile(t, 99)),
         "edit_ms_p50": float(np.percentile(edit_ms, 50)),
         "upload_bytes_per_frame": float(np.mean(up_bytes)), "uploaded_chunks_per_frame": float(np.mean(up_chunks)),
@@ -428,6 +439,8 @@ def main() -> int:
                          "edits: dynamic edit streaming; cache: radiance cache vs plain accumulation")
     ap.add_argument("--scenes", type=int, default=384, help="volumes of the batch workload")
     ap.add_argument("--frames", type=int, default=600, help="frames of the edits and cache workloads")
+    ap.add_argument("--edit-api", choices=["batch", "sphere"], default="batch",
+                    help="edits workload: one DNX_shape_spheres call per frame (device) or DN_shape_sphere per edit (host)")
     ap.add_argument("--ref-samples", type=int, default=4096,
                     help="independent samples per voxel of the cache workload's FP64 reference")
     ap.add_argument("--cache-samples", type=int, default=65536, help="cache training samples per frame")

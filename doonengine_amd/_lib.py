@@ -112,6 +112,10 @@ class DNXbatchConfig(Structure):
                 ("xcdQueues", c_uint32)]
 
 
+class DNXsphereEdit(Structure):
+    _fields_ = [("voxel", DNvoxel), ("center", DNvec3), ("radius", c_float), ("flipNormals", c_uint32)]
+
+
 class DNXrcConfig(Structure):
     _fields_ = [("log2Table", c_uint32), ("nMin", c_uint32), ("nMax", c_uint32), ("extent", c_float),
                 ("lr", c_float), ("beta1", c_float), ("beta2", c_float), ("eps", c_float),
@@ -158,6 +162,7 @@ SIGNATURES = {
     "DNX_error_count": (c_uint32, []),
     "DNX_set_voxels_dense": (None, [POINTER(DNvolume), DNivec3, DNuvec3, c_void_p, c_void_p]),
     "DNX_get_voxels_dense": (None, [POINTER(DNvolume), DNivec3, DNuvec3, c_void_p, c_void_p]),
+    "DNX_shape_spheres": (c_bool, [POINTER(DNvolume), POINTER(DNXsphereEdit), c_uint32]),
     "DNX_request_all_chunks": (None, [POINTER(DNvolume)]),
     "DNX_mark_visible": (c_uint32, [POINTER(DNvolume), c_uint32]),
     "DNX_get_stream": (c_void_p, [POINTER(DNvolume)]),
@@ -197,7 +202,8 @@ SIGNATURES = {
 }
 
 STRUCT_IDS = {0: DNvolume, 1: DNchunk, 2: DNmaterial, 3: DNchunkHandle, 4: DNvoxelNode, 5: DNXcounters,
-              6: DNXtiming, 7: DNXrayHit, 8: DNXpixelHit, 9: DNvoxel, 12: DNXbatchConfig, 13: DNXrcConfig}
+              6: DNXtiming, 7: DNXrayHit, 8: DNXpixelHit, 9: DNvoxel, 12: DNXbatchConfig, 13: DNXrcConfig,
+              14: DNXsphereEdit}
 
 _lib = None
 

@@ -39,6 +39,7 @@ SOURCES = [
     ("rcache.cpp", False),
     ("kernels.hip", True),
     ("rcache.hip", True),
+    ("edits.hip", True),
 ]
 
 

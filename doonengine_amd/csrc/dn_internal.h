@@ -222,6 +222,35 @@ int dn_accumulate_f64(const float* src, double* dst, size_t n, hipStream_t strea
 int dn_rc_reference(const DNlightParams& p, uint32_t slots, uint32_t samplesPerVoxel, uint32_t seed, double* acc,
                     hipStream_t stream);
 
+// Device sphere edits (edits.hip). A batch of spheres is split into (tile,
+// sphere) pairs, numbered in the reference's order (sphere by sphere, tiles of
+// a sphere's box in z, y, x order); each touched tile lists its pairs in that
+// order. Per pair the kernel reports what the edit did to the tile's chunk.
+enum { DN_EDIT_MODIFIED = 1, DN_EDIT_ADDED = 2, DN_EDIT_REMOVED = 4 };
+struct DNeditSphere {
+    float c[3];
+    float r2, r12;                 // r*r and (r+1)*(r+1), FP32
+    uint32_t material;             // 255: carve
+    uint32_t flip;                 // carve: turn the next shell's normals inwards
+    uint32_t carveNormal;          // carve: the caller's voxel, compressed
+    uint32_t albedo;               // the caller's albedo word (fill and carve)
+};
+struct DNeditParams {
+    const DNeditSphere* spheres;
+    const uint32_t* tileIds;       // [tiles of the batch] map index
+    const uint32_t* tileInfo;      // [tiles of the batch] chunk present (bit 31) | stored voxels
+    const uint32_t* pairStart;     // [tiles of the batch + 1] into pairSphere / pairSeq
+    const uint32_t* pairSphere;    // sphere of each pair, in edit order per tile
+    const uint32_t* pairSeq;       // sequence number of each pair
+    uint8_t* events;               // [pairs] DN_EDIT_* by sequence number
+    uint32_t* tileOut;             // [tiles of the batch] present (bit 31) | modified (bit 30) | stored voxels
+    uint2* words;                  // device copy of the CPU chunk words, [map tiles][512], voxel x*64 + y*8 + z
+    uint32_t mapX, mapY, mapZ;
+};
+int dn_launch_sphere_edits(const DNeditParams& p, uint32_t numTiles, hipStream_t stream);
+int dn_launch_edit_scatter(uint2* words, const uint32_t* tiles, const uint2* in, uint32_t n, hipStream_t stream);
+int dn_launch_edit_gather(const uint2* words, const uint32_t* tiles, uint2* out, uint32_t n, hipStream_t stream);
+
 // Fills the lighting-pass parameters of `vol` for a frame (engine.cpp).
 struct DNvolume;
 bool prepare_lighting(DNvolume* vol, int numDiffuseSamples, int maxDiffuseSamples, float time, DNlightParams& p,

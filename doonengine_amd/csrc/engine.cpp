@@ -15,6 +15,7 @@
 #include <cstring>
 #include <set>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 void (*g_DN_message_callback)(DNmessageType, DNmessageSeverity, const char*) = nullptr;
@@ -151,6 +152,14 @@ struct DNgpuVolume {
     size_t dStageCap = 0;
     uint32_t* hSmall = nullptr;        // pinned readback words
 
+    // device sphere edits: device copy of the CPU chunk words (allocated on first use)
+    uint2* dEditWords = nullptr;
+    std::vector<uint8_t> editValid;    // per tile: device words equal the CPU chunk's
+    uint8_t* hEdit = nullptr;          // pinned in/out staging
+    size_t hEditCap = 0;
+    uint8_t* dEdit = nullptr;          // device in/out staging
+    size_t dEditCap = 0;
+
     // host mirror of device tile state
     std::vector<uint8_t> state;        // 0 none, 1 unloaded, 2 loaded, 3 to load
     std::vector<uint32_t> nodeStart;
@@ -244,7 +253,8 @@ void free_tile_buffers(DNgpuVolume* g)
 {
     dfree(g->dFlags); dfree(g->dLastUsed); dfree(g->dChunks); dfree(g->dOccLoaded); dfree(g->dOccPresent);
     dfree(g->dFresh); dfree(g->dLitBits); dfree(g->dVisProp); dfree(g->dRequests); dfree(g->dBlockSums);
-    dfree(g->dReqList); dfree(g->dMarkList);
+    dfree(g->dReqList); dfree(g->dMarkList); dfree(g->dEditWords);
+    g->editValid.clear();
     g->markValid = false;
     g->exposedValid = false;
 }
@@ -308,6 +318,7 @@ bool grow_pool(DNvolume* vol, uint32_t newCap)
 void mark_dirty(DNgpuVolume* g, uint32_t t)
 {
     if(t >= g->tiles) return;
+    if(!g->editValid.empty()) g->editValid[t] = 0;
     g->exposedValid = false;
     g->markValid = false;
     if(!g->dirtyMark[t]) { g->dirtyMark[t] = 1; g->dirty.push_back(t); }
@@ -730,7 +741,8 @@ extern "C" void DN_delete_volume(DNvolume* vol)
         if(g->stream) hipStreamSynchronize(g->stream);
         free_tile_buffers(g);
         dfree(g->dVoxels); dfree(g->dVoxelsOut); dfree(g->dMats); dfree(g->dNumRequests); dfree(g->dWorkCounter);
-        dfree(g->dCounters); dfree(g->dPoints); dfree(g->dStage);
+        dfree(g->dCounters); dfree(g->dPoints); dfree(g->dStage); dfree(g->dEdit);
+        if(g->hEdit) hipHostFree(g->hEdit);
         for(int s = 0; s < 2; s++)
         {
             if(g->hStage[s]) hipHostFree(g->hStage[s]);
@@ -1488,44 +1500,68 @@ extern "C" DNvoxel DN_decompress_voxel(DNcompressedVoxel voxel)
     return r;
 }
 
-// voxelShapes.c:239-333 without the transform callback.
-extern "C" void DN_shape_sphere(DNvolume* vol, DNvoxel voxel, bool flipNormals, DNvec3 c, float r)
+namespace {
+// The tile box a sphere edit visits (voxelShapes.c:248-275: floor/ceil of the
+// bounds, one more voxel each way when carving, DN_separate_position's
+// truncating split, clamped to the map) and its FP32 radii.
+struct SphereBox {
+    DNivec3 lo, hi;
+    float r2, r12;
+    bool shell;                    // carve with flipped normals: the next shell is touched too
+};
+
+SphereBox sphere_box(const DNvolume* vol, uint8_t material, bool flipNormals, DNvec3 c, float r)
 {
     DNvec3 mn = v3(c.x - r, c.y - r, c.z - r), mx = v3(c.x + r, c.y + r, c.z + r);
     DNivec3 iMin = {(int)floorf(mn.x), (int)floorf(mn.y), (int)floorf(mn.z)};
     DNivec3 iMax = {(int)ceilf(mx.x), (int)ceilf(mx.y), (int)ceilf(mx.z)};
-    if(voxel.material == DN_MATERIAL_EMPTY)
+    if(material == DN_MATERIAL_EMPTY)
     {
         iMin = DNivec3{iMin.x - 1, iMin.y - 1, iMin.z - 1};
         iMax = DNivec3{iMax.x + 1, iMax.y + 1, iMax.z + 1};
     }
-    DNivec3 mapMin, mapMax, cmin, cmax;
-    DN_separate_position(iMin, &mapMin, &cmin);
-    DN_separate_position(iMax, &mapMax, &cmax);
-    mapMin.x = std::max(mapMin.x, 0); mapMin.y = std::max(mapMin.y, 0); mapMin.z = std::max(mapMin.z, 0);
-    mapMax.x = std::min(mapMax.x, (int)vol->mapSize.x - 1);
-    mapMax.y = std::min(mapMax.y, (int)vol->mapSize.y - 1);
-    mapMax.z = std::min(mapMax.z, (int)vol->mapSize.z - 1);
-    const float r2 = r * r, r12 = (r + 1.0f) * (r + 1.0f);
-    const bool shell = voxel.material == DN_MATERIAL_EMPTY && flipNormals;
-    // A voxel is touched only when its FP32 squared distance is below r2 (or
-    // r12 for the normal-flipping shell). Rounding is monotone, so the voxel of
-    // a tile (or of a row) nearest to the centre along each axis has the
-    // smallest FP32 distance of all of them: tiles and rows whose nearest voxel
-    // fails both tests are skipped without changing which voxels are written
-    // or in what order.
-    auto touched = [&](float d2) { return d2 < r2 || (shell && d2 < r12); };
-    auto nearest = [](int base, float cc) {
-        float m = fabsf((float)base - cc);
-        for(int k = 1; k < 8; k++) m = std::min(m, fabsf((float)(base + k) - cc));
-        return m;
-    };
+    SphereBox b;
+    DNivec3 cmin, cmax;
+    DN_separate_position(iMin, &b.lo, &cmin);
+    DN_separate_position(iMax, &b.hi, &cmax);
+    b.lo.x = std::max(b.lo.x, 0); b.lo.y = std::max(b.lo.y, 0); b.lo.z = std::max(b.lo.z, 0);
+    b.hi.x = std::min(b.hi.x, (int)vol->mapSize.x - 1);
+    b.hi.y = std::min(b.hi.y, (int)vol->mapSize.y - 1);
+    b.hi.z = std::min(b.hi.z, (int)vol->mapSize.z - 1);
+    b.r2 = r * r;
+    b.r12 = (r + 1.0f) * (r + 1.0f);
+    b.shell = material == DN_MATERIAL_EMPTY && flipNormals;
+    return b;
+}
+
+// A voxel is touched only when its FP32 squared distance is below r2 (or r12
+// for the normal-flipping shell). Rounding is monotone, so the voxel of a tile
+// (or of a row) nearest to the centre along each axis has the smallest FP32
+// distance of all of them: tiles and rows whose nearest voxel fails both tests
+// can be skipped without changing which voxels are written or in what order.
+inline bool sphere_touches(const SphereBox& b, float d2) { return d2 < b.r2 || (b.shell && d2 < b.r12); }
+
+inline float nearest_offset(int base, float cc)
+{
+    float m = fabsf((float)base - cc);
+    for(int k = 1; k < 8; k++) m = std::min(m, fabsf((float)(base + k) - cc));
+    return m;
+}
+}  // namespace
+
+// voxelShapes.c:239-333 without the transform callback.
+extern "C" void DN_shape_sphere(DNvolume* vol, DNvoxel voxel, bool flipNormals, DNvec3 c, float r)
+{
+    const SphereBox b = sphere_box(vol, voxel.material, flipNormals, c, r);
+    const DNivec3 mapMin = b.lo, mapMax = b.hi;
+    const float r2 = b.r2, r12 = b.r12;
+    auto touched = [&](float d2) { return sphere_touches(b, d2); };
     for(int mZ = mapMin.z; mZ <= mapMax.z; mZ++)
     for(int mY = mapMin.y; mY <= mapMax.y; mY++)
     for(int mX = mapMin.x; mX <= mapMax.x; mX++)
     {
         DNivec3 mp = {mX, mY, mZ};
-        const float nx = nearest(mX * 8, c.x), ny = nearest(mY * 8, c.y), nz = nearest(mZ * 8, c.z);
+        const float nx = nearest_offset(mX * 8, c.x), ny = nearest_offset(mY * 8, c.y), nz = nearest_offset(mZ * 8, c.z);
         if(!touched(nx * nx + ny * ny + nz * nz)) continue;
         for(int cZ = 0; cZ < 8; cZ++)
         for(int cY = 0; cY < 8; cY++)
@@ -1563,6 +1599,211 @@ extern "C" void DN_shape_sphere(DNvolume* vol, DNvoxel voxel, bool flipNormals, 
         }
         }
     }
+}
+
+namespace {
+inline size_t align16(size_t x) { return (x + 15u) & ~(size_t)15u; }
+
+bool ensure_edit_stage(DNgpuVolume* g, size_t hostBytes, size_t devBytes)
+{
+    if(g->hEditCap < hostBytes)
+    {
+        if(g->hEdit) hipHostFree(g->hEdit);
+        g->hEdit = nullptr;
+        g->hEditCap = 0;
+        size_t cap = std::max(hostBytes + hostBytes / 2, (size_t)1 << 20);
+        if(!HIP_OK(hipHostMalloc((void**)&g->hEdit, cap, hipHostMallocDefault), "pinned edit staging")) return false;
+        g->hEditCap = cap;
+    }
+    if(g->dEditCap < devBytes)
+    {
+        dfree(g->dEdit);
+        g->dEditCap = 0;
+        size_t cap = std::max(devBytes + devBytes / 2, (size_t)1 << 20);
+        if(!dalloc(&g->dEdit, cap, "device edit staging")) return false;
+        g->dEditCap = cap;
+    }
+    return true;
+}
+}  // namespace
+
+extern "C" bool DNX_shape_spheres(DNvolume* vol, const DNXsphereEdit* edits, uint32_t n)
+{
+    if(n == 0) return true;
+    if(!vol || !edits)
+    {
+        dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR, "DNX_shape_spheres: null volume or edit list");
+        return false;
+    }
+    DNgpuVolume* g = vol->gpu;
+    const uint32_t tiles = g->tiles;
+    if(!g->dEditWords)
+    {
+        if(!dalloc(&g->dEditWords, (size_t)tiles * 512u, "device copy of the chunk words")) return false;
+        g->editValid.assign(tiles, 0);
+    }
+
+    // (tile, sphere) pairs in the order DN_shape_sphere visits them
+    struct Pair { uint32_t tile, sphere; };
+    std::vector<Pair> pairs;
+    std::vector<DNeditSphere> sph(n);
+    for(uint32_t e = 0; e < n; e++)
+    {
+        const DNXsphereEdit& in = edits[e];
+        const SphereBox b = sphere_box(vol, in.voxel.material, in.flipNormals != 0, in.center, in.radius);
+        DNeditSphere& d = sph[e];
+        d.c[0] = in.center.x; d.c[1] = in.center.y; d.c[2] = in.center.z;
+        d.r2 = b.r2; d.r12 = b.r12;
+        d.material = in.voxel.material;
+        d.flip = b.shell ? 1u : 0u;
+        const DNcompressedVoxel cv = DN_compress_voxel(in.voxel);
+        d.carveNormal = cv.normal;
+        d.albedo = cv.albedo;
+        for(int mZ = b.lo.z; mZ <= b.hi.z; mZ++)
+        for(int mY = b.lo.y; mY <= b.hi.y; mY++)
+        for(int mX = b.lo.x; mX <= b.hi.x; mX++)
+        {
+            const float nx = nearest_offset(mX * 8, in.center.x), ny = nearest_offset(mY * 8, in.center.y),
+                        nz = nearest_offset(mZ * 8, in.center.z);
+            if(!sphere_touches(b, nx * nx + ny * ny + nz * nz)) continue;
+            pairs.push_back({flat(vol, DNivec3{mX, mY, mZ}), e});
+        }
+    }
+    const uint32_t P = (uint32_t)pairs.size();
+    if(P == 0) return true;
+
+    // tiles in first-touch order, each with its pairs in edit order
+    std::unordered_map<uint32_t, uint32_t> slotOf;
+    slotOf.reserve(P);
+    std::vector<uint32_t> tileIds, pairCount;
+    for(const Pair& q : pairs)
+    {
+        auto it = slotOf.emplace(q.tile, (uint32_t)tileIds.size());
+        if(it.second) { tileIds.push_back(q.tile); pairCount.push_back(0); }
+        pairCount[it.first->second]++;
+    }
+    const uint32_t U = (uint32_t)tileIds.size();
+    std::vector<uint32_t> stale;
+    for(uint32_t u = 0; u < U; u++)
+        if(vol->map[tileIds[u]].flag != 0 && !g->editValid[tileIds[u]]) stale.push_back(tileIds[u]);
+    const uint32_t S = (uint32_t)stale.size();
+
+    // staging layout: inputs, then outputs
+    const size_t oSph = 0, oIds = align16(oSph + sizeof(DNeditSphere) * n), oInfo = align16(oIds + 4u * U),
+                 oStart = align16(oInfo + 4u * U), oPSph = align16(oStart + 4u * (U + 1)),
+                 oPSeq = align16(oPSph + 4u * P), oStale = align16(oPSeq + 4u * P),
+                 oStaleW = align16(oStale + 4u * S), inBytes = align16(oStaleW + 4096u * S);
+    const size_t oEv = inBytes, oOut = align16(oEv + P), oGIds = align16(oOut + 4u * U),
+                 oGW = align16(oGIds + 4u * U), allBytes = align16(oGW + 4096u * U);
+    if(!ensure_edit_stage(g, allBytes, allBytes)) return false;
+    uint8_t* h = g->hEdit;
+    uint8_t* d = g->dEdit;
+    memcpy(h + oSph, sph.data(), sizeof(DNeditSphere) * n);
+    memcpy(h + oIds, tileIds.data(), 4u * U);
+    uint32_t* info = (uint32_t*)(h + oInfo);
+    uint32_t* start = (uint32_t*)(h + oStart);
+    start[0] = 0;
+    for(uint32_t u = 0; u < U; u++)
+    {
+        const DNchunkHandle& hd = vol->map[tileIds[u]];
+        info[u] = hd.flag ? (0x80000000u | vol->chunks[hd.chunkIndex].numVoxels) : 0u;
+        start[u + 1] = start[u] + pairCount[u];
+    }
+    {
+        uint32_t* pSph = (uint32_t*)(h + oPSph);
+        uint32_t* pSeq = (uint32_t*)(h + oPSeq);
+        std::vector<uint32_t> cur(start, start + U);
+        for(uint32_t q = 0; q < P; q++)
+        {
+            const uint32_t k = cur[slotOf[pairs[q].tile]]++;
+            pSph[k] = pairs[q].sphere;
+            pSeq[k] = q;
+        }
+    }
+    memcpy(h + oStale, stale.data(), 4u * S);
+    for(uint32_t i = 0; i < S; i++)
+        memcpy(h + oStaleW + 4096u * i, vol->chunks[vol->map[stale[i]].chunkIndex].voxels, 4096u);
+
+    DNeditParams ep;
+    ep.spheres = (const DNeditSphere*)(d + oSph);
+    ep.tileIds = (const uint32_t*)(d + oIds);
+    ep.tileInfo = (const uint32_t*)(d + oInfo);
+    ep.pairStart = (const uint32_t*)(d + oStart);
+    ep.pairSphere = (const uint32_t*)(d + oPSph);
+    ep.pairSeq = (const uint32_t*)(d + oPSeq);
+    ep.events = d + oEv;
+    ep.tileOut = (uint32_t*)(d + oOut);
+    ep.words = g->dEditWords;
+    ep.mapX = vol->mapSize.x; ep.mapY = vol->mapSize.y; ep.mapZ = vol->mapSize.z;
+    bool ok = HIP_OK(hipMemcpyAsync(d, h, inBytes, hipMemcpyHostToDevice, g->stream), "edit upload");
+    ok = ok && dn_launch_edit_scatter(g->dEditWords, (const uint32_t*)(d + oStale), (const uint2*)(d + oStaleW), S,
+                                      g->stream) == 0;
+    ok = ok && dn_launch_sphere_edits(ep, U, g->stream) == 0;
+    ok = ok && HIP_OK(hipMemcpyAsync(h + oEv, d + oEv, oGIds - oEv, hipMemcpyDeviceToHost, g->stream), "edit readback");
+    ok = ok && HIP_OK(hipStreamSynchronize(g->stream), "sphere edits");
+    if(!ok) return false;
+
+    // replay the chunk events in the reference's order
+    const uint8_t* ev = h + oEv;
+    for(uint32_t q = 0; q < P; q++)
+    {
+        if(!ev[q]) continue;
+        const uint32_t t = pairs[q].tile;
+        const DNivec3 mp = {(int)(t % vol->mapSize.x), (int)((t / vol->mapSize.x) % vol->mapSize.y),
+                            (int)(t / (vol->mapSize.x * vol->mapSize.y))};
+        if(ev[q] & DN_EDIT_ADDED)
+        {
+            if(DN_add_chunk(vol, mp) < 0)
+            {
+                dn_msg(DN_MESSAGE_CPU_MEMORY, DN_MESSAGE_ERROR, "DNX_shape_spheres: no chunk memory for an edit");
+                return false;
+            }
+        }
+        else if(ev[q] & DN_EDIT_REMOVED)
+            DN_remove_chunk(vol, mp);
+        else
+            mark_dirty(g, t);
+    }
+    // final words of the tiles that hold a chunk and were written
+    const uint32_t* out = (const uint32_t*)(h + oOut);
+    uint32_t* gIds = (uint32_t*)(h + oGIds);
+    std::vector<uint32_t> gSlot;
+    for(uint32_t u = 0; u < U; u++)
+    {
+        const bool present = (out[u] >> 31) != 0u, modified = ((out[u] >> 30) & 1u) != 0u;
+        if(present != (vol->map[tileIds[u]].flag != 0))
+        {
+            dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR, "DNX_shape_spheres: chunk state mismatch at tile %u",
+                   tileIds[u]);
+            return false;
+        }
+        if(present && modified)
+        {
+            gIds[gSlot.size()] = tileIds[u];
+            gSlot.push_back(u);
+        }
+    }
+    const uint32_t G = (uint32_t)gSlot.size();
+    if(G)
+    {
+        ok = HIP_OK(hipMemcpyAsync(d + oGIds, h + oGIds, 4u * G, hipMemcpyHostToDevice, g->stream), "edit gather");
+        ok = ok && dn_launch_edit_gather(g->dEditWords, (const uint32_t*)(d + oGIds), (uint2*)(d + oGW), G,
+                                         g->stream) == 0;
+        ok = ok && HIP_OK(hipMemcpyAsync(h + oGW, d + oGW, 4096u * G, hipMemcpyDeviceToHost, g->stream),
+                          "edit readback");
+        ok = ok && HIP_OK(hipStreamSynchronize(g->stream), "sphere edits");
+        if(!ok) return false;
+        for(uint32_t i = 0; i < G; i++)
+        {
+            const uint32_t u = gSlot[i];
+            DNchunk& ch = vol->chunks[vol->map[tileIds[u]].chunkIndex];
+            memcpy(ch.voxels, h + oGW + 4096u * i, 4096u);
+            ch.numVoxels = out[u] & 0x3FFFFFFFu;
+            ch.updated = true;
+        }
+    }
+    for(uint32_t u = 0; u < U; u++) g->editValid[tileIds[u]] = 1;
+    return true;
 }
 
 // ===========================================================================
@@ -1862,6 +2103,7 @@ extern "C" size_t DNX_struct_size(int which)
     case 11: return offsetof(DNvolume, camPos);
     case 12: return sizeof(DNXbatchConfig);
     case 13: return sizeof(DNXrcConfig);
+    case 14: return sizeof(DNXsphereEdit);
     default: return 0;
     }
 }

@@ -123,6 +123,41 @@ class Volume:
         vox = _lib.DNvoxel(material, DNvec3(0, 0, 0), _lib.DNcolor(*albedo))
         self.lib.DN_shape_sphere(self.ptr, vox, bool(flip_normals), DNvec3(*center), float(radius))
 
+    @staticmethod
+    def sphere_edits(edits):
+        """Packs (material, albedo, center, radius, flip_normals) tuples into a
+        DNXsphereEdit array for shape_spheres."""
+        arr = (_lib.DNXsphereEdit * max(1, len(edits)))()
+        for i, (material, albedo, center, radius, flip) in enumerate(edits):
+            arr[i].voxel = _lib.DNvoxel(material, DNvec3(0, 0, 0), _lib.DNcolor(*albedo))
+            arr[i].center = DNvec3(*center)
+            arr[i].radius = float(radius)
+            arr[i].flipNormals = 1 if flip else 0
+        return arr, len(edits)
+
+    def shape_spheres(self, edits) -> None:
+        """Applies sphere edits in order on the GPU (DNX_shape_spheres); `edits` is a
+        sequence of (material, albedo, center, radius, flip_normals) or a packed
+        (array, count) pair from sphere_edits."""
+        arr, n = edits if isinstance(edits, tuple) else self.sphere_edits(edits)
+        if not self.lib.DNX_shape_spheres(self.ptr, arr, n):
+            raise RuntimeError("DNX_shape_spheres failed")
+
+    def host_state(self):
+        """Copies of the CPU map handles and chunk array (DNvolume.map / .chunks) as
+        numpy structured arrays."""
+        v = self.ptr.contents
+        tiles = v.mapSize.x * v.mapSize.y * v.mapSize.z
+        hsz, csz = ctypes.sizeof(_lib.DNchunkHandle), ctypes.sizeof(_lib.DNchunk)
+        mp = np.frombuffer(ctypes.string_at(ctypes.addressof(v.map.contents), tiles * hsz),
+                           dtype=np.dtype({"names": ["flag", "chunkIndex"], "formats": ["u1", "<u4"],
+                                           "offsets": [0, 4], "itemsize": hsz}))
+        ch = np.frombuffer(ctypes.string_at(ctypes.addressof(v.chunks.contents), v.chunkCap * csz),
+                           dtype=np.dtype({"names": ["pos", "updated", "numVoxels", "numVoxelsGpu", "voxels"],
+                                           "formats": [("<i4", 3), "u1", "<u4", "<u4", ("<u4", (512, 2))],
+                                           "offsets": [0, 12, 16, 20, 24], "itemsize": csz}))
+        return mp.copy(), ch.copy(), int(v.nextChunk)
+
     def request_all(self) -> None:
         self.lib.DNX_request_all_chunks(self.ptr)
 

@@ -233,7 +233,8 @@ void DN_shape_sphere(DNvolume* vol, DNvoxel voxel, bool flipNormals, DNvec3 c, f
 
 /* Sizes of the public structs (0 DNvolume, 1 DNchunk, 2 DNmaterial, 3 DNchunkHandle,
  * 4 DNvoxelNode, 5 DNXcounters, 6 DNXtiming, 7 DNXrayHit, 8 DNXpixelHit, 9 DNvoxel;
- * 10/11 offsets of DNvolume.numSpecularRays / camPos), for checking bindings. */
+ * 10/11 offsets of DNvolume.numSpecularRays / camPos, 12 DNXbatchConfig, 13 DNXrcConfig,
+ * 14 DNXsphereEdit), for checking bindings. */
 size_t DNX_struct_size(int which);
 
 /* Messages and errors. */
@@ -246,6 +247,22 @@ uint32_t DNX_error_count(void);
  * encoding; material 255 means empty. Same effect as DN_set_compressed_voxel per voxel. */
 void DNX_set_voxels_dense(DNvolume* vol, DNivec3 origin, DNuvec3 size,
                           const uint32_t* normalWords, const uint32_t* albedoWords);
+/* Batched sphere edits computed on the GPU. Applies the n edits in order with
+ * exactly the effect of n DN_shape_sphere calls (voxelShapes.c:239-333, no
+ * transform callback): the same voxel words, chunk-present flags, voxel
+ * counts and `updated` flags, chunks added to the same pool slots
+ * (DN_add_chunk order) and the same dirty-chunk order for the next sync. The
+ * device keeps a copy of the CPU chunk words (4 KiB per map tile, allocated on
+ * first use); tiles changed since through the edit API are re-sent on the next
+ * call. Returns false (with a message) if a device step fails. */
+typedef struct DNXsphereEdit {
+    DNvoxel voxel;                  /* material DN_MATERIAL_EMPTY carves */
+    DNvec3 center;
+    float radius;
+    uint32_t flipNormals;           /* carve only: turn the rim's normals inwards */
+} DNXsphereEdit;
+bool DNX_shape_spheres(DNvolume* vol, const DNXsphereEdit* edits, uint32_t n);
+
 /* Reads the same dense box back (DN_get_compressed_voxel per voxel). Voxels of
  * tiles without a chunk, or outside the map, read as the cleared word pair
  * (normal 0xFFFFFFFF, albedo 0). */

@@ -57,3 +57,71 @@ def test_sphere_edits_match_oracle(gpu, seed):
                           for z in range(MAP[2])])
         np.testing.assert_array_equal(flags, grid.flag.astype(bool))
         assert 0 < flags.sum() < flags.size
+
+
+def terrain(rng, upper_empty=True):
+    X, Y, Z = (8 * m for m in MAP)
+    solid = rng.random((Z, Y, X)) < 0.3
+    if upper_empty:
+        solid[:, Y // 2:, :] = False               # fills there add chunks during the batch
+    normal = np.where(solid, (rng.integers(1, 200, (Z, Y, X)).astype(np.uint32) << 24)
+                      | rng.integers(0, 1 << 24, (Z, Y, X)).astype(np.uint32), np.uint32(0xFF000000))
+    albedo = rng.integers(0, 1 << 32, (Z, Y, X), dtype=np.uint64).astype(np.uint32) & np.uint32(0xFFFFFF00)
+    return normal, albedo
+
+
+def assert_same_host(a: Volume, b: Volume, what: str):
+    ma, ca, na = a.host_state()
+    mb, cb, nb = b.host_state()
+    assert na == nb, f"nextChunk differs {what}"
+    np.testing.assert_array_equal(ma["flag"], mb["flag"], err_msg=f"map flags {what}")
+    live = ma["flag"] != 0
+    np.testing.assert_array_equal(ma["chunkIndex"][live], mb["chunkIndex"][live], err_msg=f"chunk slots {what}")
+    assert len(ca) == len(cb), f"chunk capacity differs {what}"
+    for f in ("pos", "updated", "numVoxels", "numVoxelsGpu", "voxels"):
+        np.testing.assert_array_equal(ca[f], cb[f], err_msg=f"chunk field {f} {what}")
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_batched_device_edits_match_sequential(gpu, seed):
+    """DNX_shape_spheres (device, batched) against DN_shape_sphere one by one on
+    an identical volume: the whole CPU chunk array (slots, pos, updated,
+    counts, words), the map handles and nextChunk bit for bit after every batch
+    (batches of 1 to 40 edits, chunk memory growing inside a batch, host-side
+    DN_set_voxel / DN_remove_voxel edits between batches, which the device copy
+    must pick up), then the uploaded device state after a sync."""
+    rng = np.random.default_rng(seed)
+    normal, albedo = terrain(rng)
+    edits = list(spheres(rng, 90))
+    with Volume(MAP, min_chunks=16) as seq, Volume(MAP, min_chunks=16) as bat:
+        for v in (seq, bat):
+            v.set_dense(normal, albedo)
+        assert_same_host(seq, bat, "after terrain")
+        k, sizes = 0, [1, 7, 40, 2, 25, 15]
+        for bi, size in enumerate(sizes):
+            chunk = edits[k:k + size]
+            k += size
+            for mat, alb, c, r, flip in chunk:
+                seq.shape_sphere(mat, alb, c, r, flip_normals=flip)
+            bat.shape_spheres(chunk)
+            assert_same_host(seq, bat, f"after batch {bi}")
+            # host-side edits between batches, on both volumes
+            for j in range(6):
+                mp = tuple(int(x) for x in rng.integers(0, MAP))
+                cp = tuple(int(x) for x in rng.integers(0, 8, 3))
+                if j % 3 == 2:
+                    for v in (seq, bat):
+                        v.remove_voxel(mp, cp) if v.chunk_exists(mp) else None
+                else:
+                    m = int(rng.integers(1, 200))
+                    for v in (seq, bat):
+                        v.set_voxel(mp, cp, m, (0.0, 1.0, 0.0), (9, 8, 7))
+            assert_same_host(seq, bat, f"after host edits {bi}")
+        assert k == len(edits)
+        for v in (seq, bat):
+            v.request_all()
+            v.sync()
+            v.synchronize()
+        sa, sb = seq.export_state(), bat.export_state()
+        for x, y, name in zip(sa, sb, ("map", "chunks", "voxels")):
+            np.testing.assert_array_equal(x, y, err_msg=f"device {name} after sync")
