@@ -163,6 +163,8 @@ SIGNATURES = {
     "DNX_set_voxels_dense": (None, [POINTER(DNvolume), DNivec3, DNuvec3, c_void_p, c_void_p]),
     "DNX_get_voxels_dense": (None, [POINTER(DNvolume), DNivec3, DNuvec3, c_void_p, c_void_p]),
     "DNX_shape_spheres": (c_bool, [POINTER(DNvolume), POINTER(DNXsphereEdit), c_uint32]),
+    "DN_load_volume": (POINTER(DNvolume), [ctypes.c_char_p, c_uint32]),
+    "DN_save_volume": (c_bool, [ctypes.c_char_p, POINTER(DNvolume)]),
     "DNX_request_all_chunks": (None, [POINTER(DNvolume)]),
     "DNX_mark_visible": (c_uint32, [POINTER(DNvolume), c_uint32]),
     "DNX_get_stream": (c_void_p, [POINTER(DNvolume)]),

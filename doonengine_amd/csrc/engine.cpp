@@ -1601,6 +1601,242 @@ extern "C" void DN_shape_sphere(DNvolume* vol, DNvoxel voxel, bool flipNormals, 
     }
 }
 
+// ===========================================================================
+// FILE IO (voxel.c:296-654): the .voxvol volume file. Layout: map size (3 x
+// u32), chunk capacity (u64), per chunk slot a u16 byte count and the
+// compressed chunk, 256 raw 32-byte materials, then camera, lighting and sky
+// parameters. A compressed chunk is its position (3 x i32), and for a slot in
+// use a normal palette and an albedo palette (u8 count + 3-byte entries, in
+// first-seen order, kept only while smaller than half the chunk's voxel count)
+// followed by material runs over the voxels in x-fastest order: material, run
+// length (<= 255), and per solid voxel its normal and albedo, each a palette
+// index or 3 raw bytes.
+
+namespace {
+struct Rgb { uint8_t v[3]; };
+inline bool same_rgb(const Rgb& a, const Rgb& b) { return a.v[0] == b.v[0] && a.v[1] == b.v[1] && a.v[2] == b.v[2]; }
+inline Rgb normal_rgb(uint32_t w) { return Rgb{{(uint8_t)(w >> 16), (uint8_t)(w >> 8), (uint8_t)w}}; }
+inline Rgb albedo_rgb(uint32_t w) { return Rgb{{(uint8_t)(w >> 24), (uint8_t)(w >> 16), (uint8_t)(w >> 8)}}; }
+
+int find_rgb(const Rgb* pal, int n, const Rgb& c)
+{
+    for(int i = 0; i < n; i++)
+        if(same_rgb(pal[i], c)) return i;
+    return -1;
+}
+
+// _DN_compress_chunk (voxel.c:300-430): appends the chunk's record to `out`.
+void encode_chunk(const DNvolume* vol, const DNchunk& ch, std::vector<uint8_t>& out)
+{
+    auto put = [&](const void* p, size_t n) {
+        const uint8_t* b = (const uint8_t*)p;
+        out.insert(out.end(), b, b + n);
+    };
+    put(&ch.pos, sizeof(DNivec3));
+    if(!DN_in_map_bounds(const_cast<DNvolume*>(vol), ch.pos)) return;
+    const int half = (int)(ch.numVoxels / 2);
+    Rgb npal[DN_CHUNK_LENGTH / 2], apal[DN_CHUNK_LENGTH / 2];
+    int nn = 0, na = 0;
+    for(int i = 0; i < DN_CHUNK_LENGTH; i++)
+    {
+        const DNcompressedVoxel& v = ch.voxels[i & 7][(i >> 3) & 7][i >> 6];
+        if(material_of(v.normal) == DN_MATERIAL_EMPTY) continue;
+        const Rgb n = normal_rgb(v.normal), a = albedo_rgb(v.albedo);
+        if(nn < half && find_rgb(npal, nn, n) < 0) npal[nn++] = n;
+        if(na < half && find_rgb(apal, na, a) < 0) apal[na++] = a;
+    }
+    if(!(nn < half)) nn = 0;
+    if(!(na < half)) na = 0;
+    out.push_back((uint8_t)nn);
+    put(npal, 3u * nn);
+    out.push_back((uint8_t)na);
+    put(apal, 3u * na);
+    for(int i = 0; i < DN_CHUNK_LENGTH;)
+    {
+        const uint32_t mat = material_of(ch.voxels[i & 7][(i >> 3) & 7][i >> 6].normal);
+        out.push_back((uint8_t)mat);
+        const size_t at = out.size();
+        out.push_back(0);
+        int num = 0, j = i;
+        for(; j < DN_CHUNK_LENGTH && num < 255; j++)
+        {
+            const DNcompressedVoxel& v = ch.voxels[j & 7][(j >> 3) & 7][j >> 6];
+            if(material_of(v.normal) != mat) break;
+            num++;
+            if(mat == DN_MATERIAL_EMPTY) continue;
+            const Rgb n = normal_rgb(v.normal), a = albedo_rgb(v.albedo);
+            if(nn) out.push_back((uint8_t)find_rgb(npal, nn, n)); else put(n.v, 3);
+            if(na) out.push_back((uint8_t)find_rgb(apal, na, a)); else put(a.v, 3);
+        }
+        out[at] = (uint8_t)num;
+        i = j;
+    }
+}
+
+// Bounds-checked little reader over a file image.
+struct Reader {
+    const uint8_t* p;
+    size_t n, at = 0;
+    bool ok = true;
+    bool take(void* dst, size_t k)
+    {
+        if(!ok || k > n - at) { ok = false; return false; }
+        memcpy(dst, p + at, k);
+        at += k;
+        return true;
+    }
+    uint8_t byte() { uint8_t b = 0; take(&b, 1); return b; }
+};
+
+// _DN_decompress_chunk (voxel.c:433-518) into a cleared slot. Returns false on
+// a malformed record (the reference would read past it).
+bool decode_chunk(const DNvolume* vol, const uint8_t* mem, size_t size, DNchunk& ch)
+{
+    Reader r{mem, size};
+    r.take(&ch.pos, sizeof(DNivec3));
+    ch.updated = false;
+    ch.numVoxels = 0;
+    ch.numVoxelsGpu = 0;
+    if(!r.ok) return false;
+    if(!DN_in_map_bounds(const_cast<DNvolume*>(vol), ch.pos)) return r.at == size;
+    Rgb npal[256], apal[256];
+    const int nn = r.byte();
+    r.take(npal, 3u * nn);
+    const int na = r.byte();
+    r.take(apal, 3u * na);
+    int done = 0;
+    while(r.ok && done < DN_CHUNK_LENGTH)
+    {
+        const uint32_t mat = r.byte();
+        const int num = r.byte();
+        if(done + num > DN_CHUNK_LENGTH) return false;
+        for(int i = done; i < done + num && r.ok; i++)
+        {
+            DNcompressedVoxel& v = ch.voxels[i & 7][(i >> 3) & 7][i >> 6];
+            if(mat == DN_MATERIAL_EMPTY)
+            {
+                v.normal = UINT32_MAX;
+                continue;
+            }
+            Rgb n, a;
+            if(nn) { const int k = r.byte(); if(k >= nn) return false; n = npal[k]; } else r.take(n.v, 3);
+            if(na) { const int k = r.byte(); if(k >= na) return false; a = apal[k]; } else r.take(a.v, 3);
+            v.normal = (mat << 24) | ((uint32_t)n.v[0] << 16) | ((uint32_t)n.v[1] << 8) | n.v[2];
+            v.albedo = ((uint32_t)a.v[0] << 24) | ((uint32_t)a.v[1] << 16) | ((uint32_t)a.v[2] << 8);
+            ch.numVoxels++;
+        }
+        done += num;
+    }
+    return r.ok && r.at == size;
+}
+
+// The parameter block after the materials (voxel.c:577-591), in file order.
+template <typename F> void volume_params(DNvolume* vol, F&& io)
+{
+    io(&vol->camPos, sizeof(DNvec3)); io(&vol->camOrient, sizeof(DNvec3));
+    io(&vol->camFOV, sizeof(float)); io(&vol->camViewMode, sizeof(uint32_t));
+    io(&vol->sunDir, sizeof(DNvec3)); io(&vol->sunStrength, sizeof(DNvec3));
+    io(&vol->ambientLightStrength, sizeof(DNvec3)); io(&vol->diffuseBounceLimit, sizeof(uint32_t));
+    io(&vol->specBounceLimit, sizeof(uint32_t)); io(&vol->shadowSoftness, sizeof(float));
+    io(&vol->skyGradientBot, sizeof(DNvec3)); io(&vol->skyGradientTop, sizeof(DNvec3));
+}
+}  // namespace
+
+// voxel.c:520-593. The file is read whole and every field is bounds-checked: a
+// truncated or malformed file gives an error message and NULL.
+extern "C" DNvolume* DN_load_volume(const char* filePath, unsigned int minChunks)
+{
+    FILE* f = filePath ? fopen(filePath, "rb") : nullptr;
+    if(!f)
+    {
+        dn_msg(DN_MESSAGE_FILE_IO, DN_MESSAGE_ERROR, "failed to open file \"%s\" for reading", filePath ? filePath : "");
+        return nullptr;
+    }
+    std::vector<uint8_t> data;
+    uint8_t buf[1 << 16];
+    for(size_t k; (k = fread(buf, 1, sizeof(buf), f)) > 0;) data.insert(data.end(), buf, buf + k);
+    fclose(f);
+    Reader r{data.data(), data.size()};
+    DNuvec3 mapSize;
+    uint64_t chunkCap = 0;
+    r.take(&mapSize, sizeof(DNuvec3));
+    r.take(&chunkCap, sizeof(uint64_t));
+    // every slot takes at least its 2-byte count: bounds the allocation
+    if(!r.ok || chunkCap == 0 || chunkCap > (data.size() - r.at) / 2)
+    {
+        dn_msg(DN_MESSAGE_FILE_IO, DN_MESSAGE_ERROR, "\"%s\" is not a volume file (bad header)", filePath);
+        return nullptr;
+    }
+    DNvolume* vol = DN_create_volume(mapSize, minChunks);
+    if(!vol) return nullptr;
+    if(!DN_set_max_chunks(vol, (size_t)chunkCap)) { DN_delete_volume(vol); return nullptr; }
+    if(vol->nextChunk >= vol->chunkCap) vol->nextChunk = 0;
+    for(size_t i = 0; i < chunkCap; i++)
+    {
+        uint16_t size = 0;
+        if(!r.take(&size, sizeof(uint16_t)) || size > data.size() - r.at ||
+           !decode_chunk(vol, data.data() + r.at, size, vol->chunks[i]))
+        {
+            dn_msg(DN_MESSAGE_FILE_IO, DN_MESSAGE_ERROR, "\"%s\": malformed chunk record %zu", filePath, i);
+            DN_delete_volume(vol);
+            return nullptr;
+        }
+        r.at += size;
+        if(DN_in_map_bounds(vol, vol->chunks[i].pos))
+        {
+            const uint32_t mi = flat(vol, vol->chunks[i].pos);
+            vol->map[mi].flag = 1;
+            vol->map[mi].chunkIndex = (uint32_t)i;
+            mark_dirty(vol->gpu, mi);
+        }
+    }
+    r.take(vol->materials, sizeof(DNmaterial) * DN_MAX_MATERIALS);
+    volume_params(vol, [&](void* p, size_t k) { r.take(p, k); });
+    if(!r.ok)
+    {
+        dn_msg(DN_MESSAGE_FILE_IO, DN_MESSAGE_ERROR, "\"%s\" is truncated", filePath);
+        DN_delete_volume(vol);
+        return nullptr;
+    }
+    return vol;
+}
+
+// voxel.c:595-654. The file image is built in memory and written at once.
+extern "C" bool DN_save_volume(const char* filePath, DNvolume* vol)
+{
+    std::vector<uint8_t> out, rec;
+    auto put = [&](const void* p, size_t n) {
+        const uint8_t* b = (const uint8_t*)p;
+        out.insert(out.end(), b, b + n);
+    };
+    const uint64_t cap = vol->chunkCap;
+    put(&vol->mapSize, sizeof(DNuvec3));
+    put(&cap, sizeof(uint64_t));
+    for(size_t i = 0; i < vol->chunkCap; i++)
+    {
+        rec.clear();
+        encode_chunk(vol, vol->chunks[i], rec);
+        const uint16_t size = (uint16_t)rec.size();
+        put(&size, sizeof(uint16_t));
+        put(rec.data(), rec.size());
+    }
+    put(vol->materials, sizeof(DNmaterial) * DN_MAX_MATERIALS);
+    volume_params(vol, [&](void* p, size_t k) { put(p, k); });
+    FILE* f = filePath ? fopen(filePath, "wb") : nullptr;
+    if(!f)
+    {
+        dn_msg(DN_MESSAGE_FILE_IO, DN_MESSAGE_ERROR, "failed to open file \"%s\" for writing", filePath ? filePath : "");
+        return false;
+    }
+    const bool ok = fwrite(out.data(), 1, out.size(), f) == out.size();
+    if(fclose(f) != 0 || !ok)
+    {
+        dn_msg(DN_MESSAGE_FILE_IO, DN_MESSAGE_ERROR, "failed to write \"%s\"", filePath);
+        return false;
+    }
+    return true;
+}
+
 namespace {
 inline size_t align16(size_t x) { return (x + 15u) & ~(size_t)15u; }
 

@@ -42,16 +42,30 @@ def np_to_mat(a: np.ndarray) -> DNmat4:
 class Volume:
     """One DNvolume."""
 
-    def __init__(self, map_size, min_chunks: int = 4096):
+    def __init__(self, map_size, min_chunks: int = 4096, _path: str | None = None):
         self.lib = _lib.load()
         if not self.lib.DN_init():
             raise RuntimeError("DN_init failed: no usable HIP device")
-        self.ptr = self.lib.DN_create_volume(DNuvec3(*[int(v) for v in map_size]), int(min_chunks))
-        if not self.ptr:
-            raise RuntimeError("DN_create_volume failed")
+        if _path is not None:
+            self.ptr = self.lib.DN_load_volume(str(_path).encode(), int(min_chunks))
+            if not self.ptr:
+                raise RuntimeError(f"DN_load_volume failed for {_path}")
+        else:
+            self.ptr = self.lib.DN_create_volume(DNuvec3(*[int(v) for v in map_size]), int(min_chunks))
+            if not self.ptr:
+                raise RuntimeError("DN_create_volume failed")
         self.v = self.ptr.contents
-        self.map_size = tuple(int(v) for v in map_size)
+        self.map_size = (int(self.v.mapSize.x), int(self.v.mapSize.y), int(self.v.mapSize.z))
         self.tiles = int(np.prod(self.map_size))
+
+    @classmethod
+    def load(cls, path, min_chunks: int = 128) -> "Volume":
+        """DN_load_volume: a volume read from a .voxvol file."""
+        return cls(None, min_chunks, _path=path)
+
+    def save(self, path) -> None:
+        if not self.lib.DN_save_volume(str(path).encode(), self.ptr):
+            raise RuntimeError(f"DN_save_volume failed for {path}")
 
     # -- lifetime ---------------------------------------------------------
     def close(self) -> None:

@@ -187,6 +187,11 @@ bool DN_init(void);                                                    /* voxel.
 void DN_quit(void);                                                    /* voxel.h:158 */
 DNvolume* DN_create_volume(DNuvec3 mapSize, unsigned int minChunks);   /* voxel.h:166 */
 void DN_delete_volume(DNvolume* vol);                                  /* voxel.h:170 */
+/* The .voxvol volume file (voxel.c:520-654): chunks with per-chunk palettes and
+ * material runs, materials, camera, lighting, sky. Loading checks every field:
+ * a truncated or malformed file gives an error message and NULL. */
+DNvolume* DN_load_volume(const char* filePath, unsigned int minChunks); /* voxel.h:178 */
+bool DN_save_volume(const char* filePath, DNvolume* vol);              /* voxel.h:184 */
 
 void DN_set_view_projection_matrices(DNvolume* vol, float aspectRatio, float nearPlane, float farPlane,
                                      DNmat4* view, DNmat4* projection);                       /* voxel.h:197 */
