@@ -161,6 +161,50 @@ __device__ __forceinline__ void iterate_dda(DDA& d, f3& normal)
     normal.z = fz * (float)(-d.sz);
 }
 
+// iterate_dda for a ray whose delta = |1/dir| is finite on every axis (no
+// direction component is exactly zero). Every sideDist is then finite and
+// >= +0, so GLSL's min (y < x ? y : x) is the hardware min, and
+// side + mask * delta with mask in {0, 1} is side + delta or side itself
+// (0 * finite delta = +0 and side + +0 = side): the same values as iterate_dda
+// in fewer operations. With a zero direction component delta is infinite and
+// the reference's 0 * delta is NaN, which its arithmetic carries on; such rays
+// keep iterate_dda.
+// The side distances are also never -0 (the start value is a product of two
+// non-negative terms whose first factor cancels to +0, not -0), so their bit
+// patterns order as unsigned integers exactly as the values do: the minima and
+// comparisons below are integer operations on the bits.
+__device__ __forceinline__ void iterate_dda_finite(DDA& d, f3& normal)
+{
+    f3 sd = d.side;
+    const uint32_t bx = __float_as_uint(sd.x), by = __float_as_uint(sd.y), bz = __float_as_uint(sd.z);
+    bool mx = bx <= min(by, bz);
+    bool my = by <= min(bz, bx);
+    bool mz = bz <= min(bx, by);
+    d.side.x = mx ? sd.x + d.delta.x : sd.x;
+    d.side.y = my ? sd.y + d.delta.y : sd.y;
+    d.side.z = mz ? sd.z + d.delta.z : sd.z;
+    d.px += mx ? d.sx : 0; d.py += my ? d.sy : 0; d.pz += mz ? d.sz : 0;
+    normal.x = (mx ? 1.0f : 0.0f) * (float)(-d.sx);
+    normal.y = (my ? 1.0f : 0.0f) * (float)(-d.sy);
+    normal.z = (mz ? 1.0f : 0.0f) * (float)(-d.sz);
+}
+
+__device__ __forceinline__ bool all_finite(f3 v)
+{
+    return fabsf(v.x) < INFINITY && fabsf(v.y) < INFINITY && fabsf(v.z) < INFINITY;
+}
+
+// Cell tests as unsigned compares: a negative coordinate wraps above any bound.
+__device__ __forceinline__ bool in_chunk_cell(const DDA& d)
+{
+    return max(max((uint32_t)d.px, (uint32_t)d.py), (uint32_t)d.pz) < 8u;
+}
+
+__device__ __forceinline__ bool in_map_cell(const DDA& d, uint32_t X, uint32_t Y, uint32_t Z)
+{
+    return (uint32_t)d.px < X && (uint32_t)d.py < Y && (uint32_t)d.pz < Z;
+}
+
 // Per-ray mutable state that the shaders keep in invocation globals
 // (voxelShared.comp:321-325), plus per-lane counters.
 struct RayCtx {
@@ -208,8 +252,32 @@ __device__ __forceinline__ uint32_t rank_row(const uint32_t* row, const uint32_t
 // this build (no raster compositing), so the depth test at :363 always passes.
 // LDSM: the chunk's occupancy mask goes to the lane's LDS row (c.maskRow) on
 // entry and each voxel trip reads one word of it, instead of selecting the
-// word out of 16 registers by a per-lane index.
-template <bool REFRACT, bool LDSM = false>
+// word out of 16 registers by a per-lane index. FINITE: the ray's delta is
+// finite on every axis, so iterate_dda_finite applies (never with refraction,
+// which changes the direction).
+// lastSideDist of voxelShared.comp: the side distances before the last step,
+// of which the walk only ever uses the minimum (the distance at which the ray
+// entered its current cell). The finite-delta walk keeps just that minimum
+// (min is exact and order-free on finite values, and on their bits, see
+// iterate_dda_finite); the reference walk keeps the three distances and takes
+// GLSL's min when asked.
+template <bool FINITE> struct LastSide {
+    f3 v = splat(0.0f);
+    __device__ __forceinline__ void set(f3 s) { v = s; }
+    __device__ __forceinline__ void reset() { v = splat(0.0f); }
+    __device__ __forceinline__ float minc() const { return ::minc(v); }
+};
+template <> struct LastSide<true> {
+    float t = 0.0f;
+    __device__ __forceinline__ void set(f3 s)
+    {
+        t = __uint_as_float(min(min(__float_as_uint(s.x), __float_as_uint(s.y)), __float_as_uint(s.z)));
+    }
+    __device__ __forceinline__ void reset() { t = 0.0f; }
+    __device__ __forceinline__ float minc() const { return t; }
+};
+
+template <bool REFRACT, bool LDSM = false, bool FINITE = false>
 __device__ bool step_chunk(const DNsceneView& s, RayCtx& c, uint32_t mapIndex, f3& dir, f3& inv, f3& p,
                            bool ignoreFirst, f3& hitNormal, DVoxel& vox, f3& colorAdd, float& colorMult,
                            bool& refracted)
@@ -233,8 +301,9 @@ __device__ bool step_chunk(const DNsceneView& s, RayCtx& c, uint32_t mapIndex, f
     }
     DDA d;
     init_dda(dir, inv, p, d);
-    f3 lsd = splat(0.0f);
-    while(d.px >= 0 && d.py >= 0 && d.pz >= 0 && d.px < 8 && d.py < 8 && d.pz < 8)
+    LastSide<FINITE> lsd;
+    static_assert(!(REFRACT && FINITE), "refraction changes the direction: no finite-delta walk");
+    while(in_chunk_cell(d))
     {
         c.sVox++;
         uint32_t li = (uint32_t)(d.px + 8 * (d.py + 8 * d.pz));
@@ -250,7 +319,7 @@ __device__ bool step_chunk(const DNsceneView& s, RayCtx& c, uint32_t mapIndex, f
             uint32_t thisID = (cw.y & 0xFFFFFF00u) | (cw.x >> 24);
             if(mat.opacity == 1.0f)
             {
-                p = p + dir * (minc(lsd) + DN_EPS);
+                p = p + dir * (lsd.minc() + DN_EPS);
                 c.hitIndex = idx;
                 return true;
             }
@@ -259,7 +328,7 @@ __device__ bool step_chunk(const DNsceneView& s, RayCtx& c, uint32_t mapIndex, f
                 float op = mat.opacity;
                 colorAdd = colorAdd + (vox.albedo * (colorMult * op)) * ld3(s.sunStrength);
                 colorMult = colorMult * (1.0f - op);
-                float rayDist = minc(lsd);
+                float rayDist = lsd.minc();
                 if(REFRACT && rayDist > 0.0f)
                 {
                     refracted = true;
@@ -268,7 +337,7 @@ __device__ bool step_chunk(const DNsceneView& s, RayCtx& c, uint32_t mapIndex, f
                     dir = vrefract(dir, n, c.lastVoxRefract / mat.refractIndex);
                     inv = vinv(dir);
                     init_dda(dir, inv, p, d);
-                    lsd = splat(0.0f);
+                    lsd.reset();
                 }
                 c.lastVoxID = thisID;
                 c.lastVoxRefract = mat.refractIndex;
@@ -279,7 +348,7 @@ __device__ bool step_chunk(const DNsceneView& s, RayCtx& c, uint32_t mapIndex, f
             if(REFRACT)
             {
                 refracted = true;
-                p = p + dir * (minc(lsd) + DN_EPS);
+                p = p + dir * (lsd.minc() + DN_EPS);
                 f3 old = dir;
                 f3 nn = -vox.normal;
                 f3 n = dot3(nn, dir) < 0.0f ? vnorm(nn) : hitNormal;
@@ -287,20 +356,21 @@ __device__ bool step_chunk(const DNsceneView& s, RayCtx& c, uint32_t mapIndex, f
                 if(dir.x == 0.0f && dir.y == 0.0f && dir.z == 0.0f) dir = old;
                 inv = vinv(dir);
                 init_dda(dir, inv, p, d);
-                lsd = splat(0.0f);
+                lsd.reset();
             }
             c.lastVoxID = 255u;
             c.lastVoxRefract = 1.0f;
         }
-        lsd = d.side;
-        iterate_dda(d, hitNormal);
+        lsd.set(d.side);
+        if(FINITE) iterate_dda_finite(d, hitNormal);
+        else iterate_dda(d, hitNormal);
         ignoreFirst = false;
     }
     return false;
 }
 
 // step_map, voxelShared.comp:421-475
-template <bool REFRACT, bool LDSM = false>
+template <bool REFRACT, bool LDSM = false, bool FINITE = false>
 __device__ bool step_map(const DNsceneView& s, RayCtx& c, f3& dir, f3 inv, f3& p, bool ignoreFirst, f3& hitNormal,
                          DVoxel& vox, f3& colorAdd, float& colorMult)
 {
@@ -310,21 +380,21 @@ __device__ bool step_map(const DNsceneView& s, RayCtx& c, f3& dir, f3 inv, f3& p
     c.sMap = 0; c.sVox = 0; c.hitIndex = 0xFFFFFFFFu;
     DDA d;
     init_dda(dir, inv, p, d);
-    f3 lsd = splat(0.0f);
+    LastSide<FINITE> lsd;
     bool hit = false;
     const int32_t X = (int32_t)s.mapX, Y = (int32_t)s.mapY, Z = (int32_t)s.mapZ;
-    while(d.px >= 0 && d.py >= 0 && d.pz >= 0 && d.px < X && d.py < Y && d.pz < Z)
+    while(in_map_cell(d, (uint32_t)X, (uint32_t)Y, (uint32_t)Z))
     {
         c.sMap++;
         uint32_t idx = (uint32_t)d.px + s.mapX * ((uint32_t)d.py + s.mapY * (uint32_t)d.pz);
         if(occ_bit(s.occLoaded, idx))
         {
             s.lastUsed[idx] = 0u;          // a plain store: no read on the ray's path
-            f3 upd = p + dir * (minc(lsd) - DN_EPS);
+            f3 upd = p + dir * (lsd.minc() - DN_EPS);
             f3 crp = (upd - mk((float)d.px, (float)d.py, (float)d.pz)) * 8.0f;
             crp = vclamp(crp, DN_EPS, 8.0f - DN_EPS);
             bool refr;
-            if(step_chunk<REFRACT, LDSM>(s, c, idx, dir, inv, crp, ignoreFirst, hitNormal, vox, colorAdd, colorMult,
+            if(step_chunk<REFRACT, LDSM, FINITE>(s, c, idx, dir, inv, crp, ignoreFirst, hitNormal, vox, colorAdd, colorMult,
                                          refr))
             {
                 p = mk((float)d.px, (float)d.py, (float)d.pz) + crp * 0.125f;
@@ -335,7 +405,7 @@ __device__ bool step_map(const DNsceneView& s, RayCtx& c, f3& dir, f3 inv, f3& p
             {
                 p = mk((float)d.px, (float)d.py, (float)d.pz) + crp * 0.125f;
                 init_dda(dir, inv, p, d);
-                lsd = splat(0.0f);
+                lsd.reset();
             }
         }
         else if(s.hasUnloaded && !requested && occ_bit(s.occPresent, idx))
@@ -343,8 +413,9 @@ __device__ bool step_map(const DNsceneView& s, RayCtx& c, f3& dir, f3 inv, f3& p
             s.flags[idx] = 3u;
             requested = true;
         }
-        lsd = d.side;
-        iterate_dda(d, hitNormal);
+        lsd.set(d.side);
+        if(FINITE) iterate_dda_finite(d, hitNormal);
+        else iterate_dda(d, hitNormal);
         ignoreFirst = false;
     }
     c.cRays++;

@@ -16,6 +16,17 @@ __device__ __forceinline__ bool light_trace(const DNsceneView& s, RayCtx& c, f3 
     return step_map<false, true>(s, c, dir, inv, p, true, tn, vox, ca, cm);
 }
 
+// light_trace for the frame-uniform shadow directions (the same for every
+// voxel, so the test is uniform across the wave): the finite-delta walk when
+// no direction component is zero.
+__device__ __forceinline__ bool light_trace_uniform(const DNsceneView& s, RayCtx& c, f3 dir, f3 inv, f3& p,
+                                                    DVoxel& vox, f3& ca, float& cm)
+{
+    f3 tn = splat(0.0f);
+    if(all_finite(inv)) return step_map<false, true, true>(s, c, dir, inv, p, true, tn, vox, ca, cm);
+    return step_map<false, true>(s, c, dir, inv, p, true, tn, vox, ca, cm);
+}
+
 // The calling lane's row of a block's mask rows.
 __device__ __forceinline__ uint32_t* lane_mask_row(uint32_t* rows)
 {
@@ -52,7 +63,7 @@ __device__ void shadow_ray(LCtx& L, f3 p, uint32_t sample, f3& color)
     for(uint32_t k = 0; k < P.numDirectRays; k++)
     {
         f3 pp = p;
-        if(!light_trace(*L.s, *L.c, ld3(P.directRays[k]), ld3(P.directRays[k] + 3), pp, hv, ca, cm))
+        if(!light_trace_uniform(*L.s, *L.c, ld3(P.directRays[k]), ld3(P.directRays[k] + 3), pp, hv, ca, cm))
             acc = acc + (L.sunStrength * cm + ca);
     }
     color = color + acc / (float)P.numDirectRays;
