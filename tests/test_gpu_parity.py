@@ -238,3 +238,26 @@ def test_transparent_draw_parity(gpu):
     assert np.array_equal(px["voxelSteps"], opx["voxelSteps"])
     np.testing.assert_allclose(rgba[..., :3], orgba[..., :3], rtol=2e-6, atol=1e-7)
     vol.close()
+
+
+def test_lighting_parity_direct_ray_with_zero_component(gpu):
+    """A direct-ray direction with an exactly-zero component (infinite delta on
+    that axis, where the reference's 0 * delta is NaN) takes the reference walk;
+    the other directions take the finite-delta walk. Both against the oracle.
+    Sun along -x with softness 1e-4 turns the first direct point (0, 1, 0) into
+    normalize(-1e-4, 1, 0) + 1e-4 = (0, 1.0001, 1e-4)."""
+    sc = scenes.config_small(seed=14)
+    o = oracle.load("f32")
+    vol = load_scene(sc)
+    vol.v.numDirectRays = 8
+    vol.v.sunDir.x, vol.v.sunDir.y, vol.v.sunDir.z = -1.0, 0.0, 0.0
+    vol.v.shadowSoftness = 1e-4
+    q = vol.ray_points(1).astype(np.float32)
+    f = np.float32
+    v = f(-1.0) * f(1e-4) + q[:, 0], f(0.0) * f(1e-4) + q[:, 1], f(0.0) * f(1e-4) + q[:, 2]
+    inv = f(1.0) / np.sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2])
+    dirs = np.stack([c * inv + f(1e-4) for c in v], axis=1)
+    assert (dirs == 0).any(axis=1).sum() >= 1 and (dirs != 0).all(axis=1).sum() >= 4
+    for fr in range(2):
+        run_frame_and_compare(vol, sc, o, fr, 400.0 + fr / 60.0, sc.width, sc.height)
+    vol.close()
