@@ -163,6 +163,7 @@ SIGNATURES = {
     "DNX_set_voxels_dense": (None, [POINTER(DNvolume), DNivec3, DNuvec3, c_void_p, c_void_p]),
     "DNX_get_voxels_dense": (None, [POINTER(DNvolume), DNivec3, DNuvec3, c_void_p, c_void_p]),
     "DNX_shape_spheres": (c_bool, [POINTER(DNvolume), POINTER(DNXsphereEdit), c_uint32]),
+    "DNX_set_sky_cubemap": (c_bool, [POINTER(DNvolume), c_void_p, c_uint32]),
     "DN_load_volume": (POINTER(DNvolume), [ctypes.c_char_p, c_uint32]),
     "DN_save_volume": (c_bool, [ctypes.c_char_p, POINTER(DNvolume)]),
     "DNX_request_all_chunks": (None, [POINTER(DNvolume)]),

@@ -566,8 +566,44 @@ __device__ __forceinline__ bool step_map_flat(const DNsceneView& s, RayCtx& c, f
     return hit;
 }
 
+// texture(samplerCube, dir) with GL_LINEAR filtering, GL_CLAMP_TO_EDGE and
+// RGB8 texels, as the reference creates its sky cubemap (main.c:335-353).
+// Face selection and (s, t) follow the OpenGL cube map table (major axis
+// +X: sc = -z, tc = -y; -X: sc = z, tc = -y; +Y: sc = x, tc = z; -Y: sc = x,
+// tc = -z; +Z: sc = x, tc = -y; -Z: sc = -x, tc = -y; s = (sc/|ma| + 1)/2);
+// on equal magnitudes x wins over y over z (the specification leaves it to
+// the implementation). Texel (i, j) of a face is row j of its upload.
+__device__ __forceinline__ f3 cube_texel(const uchar4* face, uint32_t n, int32_t i, int32_t j)
+{
+    i = i < 0 ? 0 : (i > (int32_t)n - 1 ? (int32_t)n - 1 : i);
+    j = j < 0 ? 0 : (j > (int32_t)n - 1 ? (int32_t)n - 1 : j);
+    const uchar4 t = face[(uint32_t)j * n + (uint32_t)i];
+    return mk((float)t.x / 255.0f, (float)t.y / 255.0f, (float)t.z / 255.0f);
+}
+
+__device__ __forceinline__ f3 cube_sample(const uchar4* cube, uint32_t n, f3 d)
+{
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    uint32_t f;
+    float sc, tc, ma;
+    if(ax >= ay && ax >= az) { f = d.x >= 0.0f ? 0u : 1u; sc = d.x >= 0.0f ? -d.z : d.z; tc = -d.y; ma = ax; }
+    else if(ay >= az) { f = d.y >= 0.0f ? 2u : 3u; sc = d.x; tc = d.y >= 0.0f ? d.z : -d.z; ma = ay; }
+    else { f = d.z >= 0.0f ? 4u : 5u; sc = d.z >= 0.0f ? d.x : -d.x; tc = -d.y; ma = az; }
+    const float s = (sc / ma + 1.0f) * 0.5f, t = (tc / ma + 1.0f) * 0.5f;
+    const float u = s * (float)n - 0.5f, v = t * (float)n - 0.5f;
+    const float fu = floorf(u), fv = floorf(v);
+    const float a = u - fu, b = v - fv;
+    const int32_t i0 = (int32_t)fu, j0 = (int32_t)fv;
+    const uchar4* face = cube + (size_t)f * n * n;
+    const f3 c00 = cube_texel(face, n, i0, j0), c10 = cube_texel(face, n, i0 + 1, j0);
+    const f3 c01 = cube_texel(face, n, i0, j0 + 1), c11 = cube_texel(face, n, i0 + 1, j0 + 1);
+    return ((c00 * ((1.0f - a) * (1.0f - b)) + c10 * (a * (1.0f - b))) + c01 * ((1.0f - a) * b)) + c11 * (a * b);
+}
+
+// sky_color, voxelShared.comp:289-295
 __device__ __forceinline__ f3 sky_color(const DNsceneView& s, f3 dir)
 {
+    if(s.cube) return cube_sample(s.cube, s.cubeSize, dir);
     return vmix(ld3(s.skyBot), ld3(s.skyTop), dir.y * 0.5f + 1.0f);
 }
 

@@ -44,6 +44,8 @@ struct DNsceneView {
     uint32_t hasUnloaded;        // any tile in state 1/3: rays may request chunks
     float sunStrength[3];
     float skyBot[3], skyTop[3];
+    const uchar4* cube;          // sky cubemap: 6 faces (+X, -X, +Y, -Y, +Z, -Z) of cubeSize^2 RGBA8, or null
+    uint32_t cubeSize;
 };
 
 // Limits of the per-frame tables passed by value to the lighting kernel.

@@ -152,6 +152,9 @@ struct DNgpuVolume {
     size_t dStageCap = 0;
     uint32_t* hSmall = nullptr;        // pinned readback words
 
+    uchar4* dCube = nullptr;           // sky cubemap faces (DNX_set_sky_cubemap)
+    uint32_t cubeSize = 0;
+
     // device sphere edits: device copy of the CPU chunk words (allocated on first use)
     uint2* dEditWords = nullptr;
     std::vector<uint8_t> editValid;    // per tile: device words equal the CPU chunk's
@@ -568,6 +571,8 @@ DNsceneView scene_view(DNvolume* vol)
     s.sunStrength[0] = vol->sunStrength.x; s.sunStrength[1] = vol->sunStrength.y; s.sunStrength[2] = vol->sunStrength.z;
     s.skyBot[0] = vol->skyGradientBot.x; s.skyBot[1] = vol->skyGradientBot.y; s.skyBot[2] = vol->skyGradientBot.z;
     s.skyTop[0] = vol->skyGradientTop.x; s.skyTop[1] = vol->skyGradientTop.y; s.skyTop[2] = vol->skyGradientTop.z;
+    s.cube = (vol->useCubemap && g->dCube) ? g->dCube : nullptr;
+    s.cubeSize = g->cubeSize;
     return s;
 }
 
@@ -741,7 +746,7 @@ extern "C" void DN_delete_volume(DNvolume* vol)
         if(g->stream) hipStreamSynchronize(g->stream);
         free_tile_buffers(g);
         dfree(g->dVoxels); dfree(g->dVoxelsOut); dfree(g->dMats); dfree(g->dNumRequests); dfree(g->dWorkCounter);
-        dfree(g->dCounters); dfree(g->dPoints); dfree(g->dStage); dfree(g->dEdit);
+        dfree(g->dCounters); dfree(g->dPoints); dfree(g->dStage); dfree(g->dEdit); dfree(g->dCube);
         if(g->hEdit) hipHostFree(g->hEdit);
         for(int s = 0; s < 2; s++)
         {
@@ -1131,7 +1136,9 @@ extern "C" void DN_draw(DNvolume* vol, DNtexture out, DNmat4 view, DNmat4 projec
     DNgpuVolume* g = vol->gpu;
     if(rasterColorTexture >= 0 || rasterDepthTexture >= 0)
         dn_msg(DN_MESSAGE_SHADER, DN_MESSAGE_ERROR, "raster compositing is not supported; drawing without it");
-    if(vol->useCubemap) dn_msg(DN_MESSAGE_SHADER, DN_MESSAGE_ERROR, "cubemap sky is not supported; using the gradient");
+    if(vol->useCubemap && !g->dCube)
+        dn_msg(DN_MESSAGE_SHADER, DN_MESSAGE_ERROR,
+               "useCubemap is set but no cubemap was given (DNX_set_sky_cubemap); using the gradient");
     if(!out.data || out.width == 0 || out.height == 0)
     {
         dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR, "invalid output texture");
@@ -2089,6 +2096,31 @@ extern "C" void DNX_get_voxels_dense(DNvolume* vol, DNivec3 origin, DNuvec3 size
         normalWords[k] = v.normal;
         albedoWords[k] = v.albedo;
     }
+}
+
+extern "C" bool DNX_set_sky_cubemap(DNvolume* vol, const uint8_t* faces, uint32_t size)
+{
+    DNgpuVolume* g = vol->gpu;
+    hipStreamSynchronize(g->stream);
+    dfree(g->dCube);
+    g->cubeSize = 0;
+    if(!faces || size == 0) return true;
+    if(size > 16384u)
+    {
+        dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR, "cubemap faces of %u texels are too large", size);
+        return false;
+    }
+    const size_t texels = (size_t)6 * size * size;
+    std::vector<uchar4> rgba(texels);
+    for(size_t i = 0; i < texels; i++) rgba[i] = make_uchar4(faces[3 * i], faces[3 * i + 1], faces[3 * i + 2], 255);
+    if(!dalloc(&g->dCube, texels, "sky cubemap") ||
+       !HIP_OK(hipMemcpy(g->dCube, rgba.data(), sizeof(uchar4) * texels, hipMemcpyHostToDevice), "sky cubemap upload"))
+    {
+        dfree(g->dCube);
+        return false;
+    }
+    g->cubeSize = size;
+    return true;
 }
 
 extern "C" void DNX_request_all_chunks(DNvolume* vol) { vol->gpu->requestAll = true; }

@@ -172,6 +172,21 @@ class Volume:
                                            "offsets": [0, 12, 16, 20, 24], "itemsize": csz}))
         return mp.copy(), ch.copy(), int(v.nextChunk)
 
+    def set_sky_cubemap(self, faces) -> None:
+        """DNX_set_sky_cubemap: faces (6, n, n, 3) uint8 in GL order (+X, -X, +Y, -Y,
+        +Z, -Z), row 0 first; None removes the cubemap. Sets useCubemap."""
+        if faces is None:
+            self.lib.DNX_set_sky_cubemap(self.ptr, None, 0)
+            self.cube_faces = None
+            self.v.useCubemap = False
+            return
+        f = np.ascontiguousarray(faces, np.uint8)
+        assert f.ndim == 4 and f.shape[0] == 6 and f.shape[1] == f.shape[2] and f.shape[3] == 3
+        if not self.lib.DNX_set_sky_cubemap(self.ptr, f.ctypes.data, f.shape[1]):
+            raise RuntimeError("DNX_set_sky_cubemap failed")
+        self.cube_faces = f
+        self.v.useCubemap = True
+
     def request_all(self) -> None:
         self.lib.DNX_request_all_chunks(self.ptr)
 

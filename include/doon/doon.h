@@ -163,8 +163,8 @@ typedef struct DNvolume {           /* voxel.h:97-142 */
     uint32_t specBounceLimit;       /* READ-WRITE */
     float shadowSoftness;           /* READ-WRITE */
 
-    bool useCubemap;                /* READ-WRITE | must stay false: no cubemap on this path */
-    uint32_t cubemapTex;            /* READ-WRITE | unused */
+    bool useCubemap;                /* READ-WRITE | sample the sky from the cubemap given with DNX_set_sky_cubemap */
+    uint32_t cubemapTex;            /* READ-WRITE | unused (a GL texture handle in the reference) */
     DNvec3 skyGradientBot;          /* READ-WRITE */
     DNvec3 skyGradientTop;          /* READ-WRITE */
 
@@ -252,6 +252,13 @@ uint32_t DNX_error_count(void);
  * encoding; material 255 means empty. Same effect as DN_set_compressed_voxel per voxel. */
 void DNX_set_voxels_dense(DNvolume* vol, DNivec3 origin, DNuvec3 size,
                           const uint32_t* normalWords, const uint32_t* albedoWords);
+/* Sky cubemap for useCubemap (voxelShared.comp:291-292, main.c:335-353): six
+ * square RGB8 faces of size x size texels in GL order (+X, -X, +Y, -Y, +Z, -Z),
+ * each face's rows in glTexImage2D order (row 0 first), sampled like the
+ * reference's GL_LINEAR / GL_CLAMP_TO_EDGE texture. NULL or size 0 removes it.
+ * Replaces the reference's GL texture handle vol->glCubemapTex. */
+bool DNX_set_sky_cubemap(DNvolume* vol, const uint8_t* faces, uint32_t size);
+
 /* Batched sphere edits computed on the GPU. Applies the n edits in order with
  * exactly the effect of n DN_shape_sphere calls (voxelShapes.c:239-333, no
  * transform callback): the same voxel words, chunk-present flags, voxel

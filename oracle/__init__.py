@@ -27,7 +27,8 @@ class ORcounters(Structure):
 class ORscene(Structure):
     _fields_ = [("map", c_void_p), ("chunks", c_void_p), ("voxels", c_void_p), ("mats", c_void_p),
                 ("mapSize", c_uint32 * 3), ("sunStrength", c_float * 3), ("skyBot", c_float * 3),
-                ("skyTop", c_float * 3), ("flagsOut", c_void_p), ("lastUsedOut", c_void_p)]
+                ("skyTop", c_float * 3), ("flagsOut", c_void_p), ("lastUsedOut", c_void_p), ("cube", c_void_p),
+                ("cubeSize", c_uint32)]
 
 
 class ORdrawParams(Structure):
@@ -119,8 +120,25 @@ class Oracle:
         s.skyTop[:] = sky_top
         s.flagsOut = None
         s.lastUsedOut = None
+        s.cube = None
+        s.cubeSize = 0
         s._keep = keep
         return s
+
+    def set_cubemap(self, scene: ORscene, faces) -> None:
+        """Sky cubemap of a scene: faces (6, n, n, 3) uint8 in GL order, None for the gradient."""
+        if faces is None:
+            scene.cube, scene.cubeSize = None, 0
+            return
+        f = np.ascontiguousarray(faces, np.uint8)
+        scene._cube = f
+        scene.cube, scene.cubeSize = f.ctypes.data, f.shape[1]
+
+    def sky_color(self, scene: ORscene, direction) -> np.ndarray:
+        d = np.ascontiguousarray(direction, np.float32)
+        out = np.zeros(3, self.real)
+        self.lib.or_sky_color(ctypes.byref(scene), d.ctypes.data_as(c_void_p), out.ctypes.data_as(c_void_p))
+        return out
 
     # -- primitives -------------------------------------------------------
     def rank(self, chunk: np.ndarray, x: int, y: int, z: int) -> int:

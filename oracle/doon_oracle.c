@@ -142,6 +142,8 @@ typedef struct ORscene {
     float skyBot[3], skyTop[3];
     uint32_t* flagsOut;           /* tiles words: receives request writes (flags = 3), may be NULL */
     uint32_t* lastUsedOut;        /* tiles words: receives lastUsed = 0, may be NULL */
+    const uint8_t* cube;          /* sky cubemap: 6 faces (+X, -X, +Y, -Y, +Z, -Z) of cubeSize^2 RGB8, or NULL */
+    uint32_t cubeSize;
 } ORscene;
 
 typedef struct ORcounters {
@@ -414,9 +416,53 @@ static int step_map(const ORscene* s, ORthread* t, v3* dir, v3 inv, v3* p, int i
     return hit;
 }
 
+/* texture(samplerCube, dir) for the reference's sky cubemap (main.c:335-353:
+ * GL_LINEAR, GL_CLAMP_TO_EDGE, RGB8 faces uploaded as GL_TEXTURE_CUBE_MAP_POSITIVE_X
+ * + i). Cube map face selection after the OpenGL specification's table
+ * (major axis ma; +X: sc = -z, tc = -y; -X: sc = z, tc = -y; +Y: sc = x,
+ * tc = z; -Y: sc = x, tc = -z; +Z: sc = x, tc = -y; -Z: sc = -x, tc = -y;
+ * s = (sc/|ma| + 1)/2, t = (tc/|ma| + 1)/2), ties to x, then y (the
+ * specification leaves ties to the implementation); GL_LINEAR over texel
+ * centres, u = s*n - 1/2, weights (1-a)(1-b), a(1-b), (1-a)b, ab, indices
+ * clamped to the face. UNORM8 texels read as c/255. */
+static v3 cube_texel(const uint8_t* face, uint32_t n, int32_t i, int32_t j)
+{
+    i = i < 0 ? 0 : (i > (int32_t)n - 1 ? (int32_t)n - 1 : i);
+    j = j < 0 ? 0 : (j > (int32_t)n - 1 ? (int32_t)n - 1 : j);
+    const uint8_t* t = face + 3u * ((uint32_t)j * n + (uint32_t)i);
+    return V((real)t[0] / RL(255.0), (real)t[1] / RL(255.0), (real)t[2] / RL(255.0));
+}
+
+static v3 cube_sample(const uint8_t* cube, uint32_t n, v3 d)
+{
+    real ax = r_fabs(d.x), ay = r_fabs(d.y), az = r_fabs(d.z), sc, tc, ma;
+    uint32_t f;
+    if(ax >= ay && ax >= az) { f = d.x >= 0 ? 0u : 1u; sc = d.x >= 0 ? -d.z : d.z; tc = -d.y; ma = ax; }
+    else if(ay >= az) { f = d.y >= 0 ? 2u : 3u; sc = d.x; tc = d.y >= 0 ? d.z : -d.z; ma = ay; }
+    else { f = d.z >= 0 ? 4u : 5u; sc = d.z >= 0 ? d.x : -d.x; tc = -d.y; ma = az; }
+    real s = (sc / ma + RL(1.0)) * RL(0.5), t = (tc / ma + RL(1.0)) * RL(0.5);
+    real u = s * (real)n - RL(0.5), v = t * (real)n - RL(0.5);
+    real fu = r_floor(u), fv = r_floor(v), a = u - fu, b = v - fv;
+    int32_t i0 = (int32_t)fu, j0 = (int32_t)fv;
+    const uint8_t* face = cube + (size_t)f * n * n * 3u;
+    v3 c00 = cube_texel(face, n, i0, j0), c10 = cube_texel(face, n, i0 + 1, j0);
+    v3 c01 = cube_texel(face, n, i0, j0 + 1), c11 = cube_texel(face, n, i0 + 1, j0 + 1);
+    return vadd(vadd(vadd(vmuls(c00, (RL(1.0) - a) * (RL(1.0) - b)), vmuls(c10, a * (RL(1.0) - b))),
+                     vmuls(c01, (RL(1.0) - a) * b)), vmuls(c11, a * b));
+}
+
+/* sky_color, voxelShared.comp:289-295 */
 static inline v3 sky_color(const ORscene* s, v3 dir)
 {
+    if(s->cube) return cube_sample(s->cube, s->cubeSize, dir);
     return vmix(vf(s->skyBot), vf(s->skyTop), dir.y * RL(0.5) + RL(1.0));
+}
+
+/* sky_color alone, for tests: out = rgb */
+void or_sky_color(const ORscene* s, const float* dir, real* out)
+{
+    v3 c = sky_color(s, vf(dir));
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
 }
 
 /* ------------------------------------------------------------------------- */

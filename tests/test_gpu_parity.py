@@ -35,6 +35,8 @@ def oracle_scene(o, vol, sc, state=None):
     s = o.scene(sc.map_size, m, c, v, mats, sun=tuple(getattr(vol.v.sunStrength, k) for k in "xyz"),
                 sky_bot=tuple(getattr(vol.v.skyGradientBot, k) for k in "xyz"),
                 sky_top=tuple(getattr(vol.v.skyGradientTop, k) for k in "xyz"))
+    if vol.v.useCubemap and getattr(vol, "cube_faces", None) is not None:
+        o.set_cubemap(s, vol.cube_faces)
     return s, (m, c, v)
 
 
@@ -260,4 +262,31 @@ def test_lighting_parity_direct_ray_with_zero_component(gpu):
     assert (dirs == 0).any(axis=1).sum() >= 1 and (dirs != 0).all(axis=1).sum() >= 4
     for fr in range(2):
         run_frame_and_compare(vol, sc, o, fr, 400.0 + fr / 60.0, sc.width, sc.height)
+    vol.close()
+
+
+def test_cubemap_sky_draw_and_lighting_parity(gpu):
+    """Sky from a cubemap (DNX_set_sky_cubemap) in the draw's background and in
+    the lighting pass's mirror reflections (reflectType 1 specular rays that
+    miss), against the oracle's GL-rules sampler."""
+    sc = scenes.config_small(seed=15)
+    sc.materials[1].specular = 1.0
+    sc.materials[1].shininess = 100
+    sc.materials[1].reflect_type = 1
+    o = oracle.load("f32")
+    vol = load_scene(sc)
+    rng = np.random.default_rng(15)
+    vol.set_sky_cubemap(rng.integers(0, 256, (6, 16, 16, 3), dtype=np.uint8))
+    vol.v.numSpecularRays = 15
+    s, _ = oracle_scene(o, vol, sc)
+    rgba, px, view, proj = draw_gpu(vol, sc)
+    from doonengine_amd.volume import draw_matrices
+    iv, ivp = draw_matrices(view, proj)
+    orgba, opx, _ = o.draw(s, mat_to_np(iv), mat_to_np(ivp), sc.width, sc.height)
+    sky = px["voxelIndex"] == 0xFFFFFFFF
+    assert sky.mean() > 0.05
+    assert np.array_equal(px["voxelIndex"], opx["voxelIndex"])
+    np.testing.assert_allclose(rgba[..., :3], orgba[..., :3], rtol=2e-6, atol=1e-7)
+    for f in range(2):
+        run_frame_and_compare(vol, sc, o, f, 500.0 + f / 60.0, sc.width, sc.height)
     vol.close()
