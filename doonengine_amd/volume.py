@@ -187,6 +187,13 @@ class Volume:
         self.cube_faces = f
         self.v.useCubemap = True
 
+    def pick(self, direction, origin, max_steps: int):
+        """DN_step_map (the CPU pick ray): (hit, voxel position, face normal, DNvoxel)."""
+        pos, nrm, vox = DNivec3(), DNivec3(), _lib.DNvoxel()
+        hit = self.lib.DN_step_map(self.ptr, DNvec3(*direction), DNvec3(*origin), int(max_steps), ctypes.byref(pos),
+                                   ctypes.byref(vox), ctypes.byref(nrm))
+        return bool(hit), (pos.x, pos.y, pos.z), (nrm.x, nrm.y, nrm.z), vox
+
     def request_all(self) -> None:
         self.lib.DNX_request_all_chunks(self.ptr)
 

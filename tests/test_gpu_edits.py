@@ -125,3 +125,45 @@ def test_batched_device_edits_match_sequential(gpu, seed):
         sa, sb = seq.export_state(), bat.export_state()
         for x, y, name in zip(sa, sb, ("map", "chunks", "voxels")):
             np.testing.assert_array_equal(x, y, err_msg=f"device {name} after sync")
+
+
+def test_pick_ray_matches_oracle(gpu):
+    """DN_step_map, the CPU pick ray (voxel.c:1195-1272), against the oracle's
+    restatement on the same edited volume: hit flag, voxel position, face normal
+    of the last step and the voxel read back, for random, axis-parallel, exact
+    diagonal (tie) and outside-the-map rays with several step limits."""
+    from oracle.pick import step_map
+    rng = np.random.default_rng(9)
+    normal, albedo = terrain(rng, upper_empty=False)
+    o = load("f32")
+    grid = o.edit_grid(MAP)
+    grid.set_dense(normal, albedo)
+    X, Y, Z = (8 * m for m in MAP)
+    with Volume(MAP, min_chunks=16) as vol:
+        vol.set_dense(normal, albedo)
+        for mat, alb, c, r, flip in list(spheres(rng, 12)):
+            vol.shape_sphere(mat, alb, c, r, flip_normals=flip)
+            grid.shape_sphere(mat, alb, c, r, flip_normals=flip)
+        rays = []
+        for k in range(600):
+            org = rng.uniform(-0.3, 1.3, 3) * np.array(MAP)
+            if k % 4 == 0:
+                d = np.zeros(3); d[rng.integers(0, 3)] = rng.choice([-1.0, 1.0])
+            elif k % 4 == 1:
+                d = rng.choice([-1.0, 1.0], 3)
+                org = (np.floor(org * 8) + 0.5) / 8            # voxel centres: exact ties
+            else:
+                d = rng.normal(size=3)
+            rays.append((d.astype(np.float32), org.astype(np.float32), int(rng.choice([1, 7, 40, 200]))))
+        hits = 0
+        for d, org, steps in rays:
+            g = vol.pick(tuple(float(v) for v in d), tuple(float(v) for v in org), steps)
+            hit, pos, n, idx = step_map(grid.normal, grid.flag, d, org, steps)
+            assert g[0] == hit and g[2] == n, (d, org, steps)
+            if hit:
+                hits += 1
+                assert g[1] == pos
+                w, a = int(grid.normal[idx]), int(grid.albedo[idx])
+                assert g[3].material == w >> 24
+                assert (g[3].albedo.r, g[3].albedo.g, g[3].albedo.b) == ((a >> 24) & 255, (a >> 16) & 255, (a >> 8) & 255)
+        assert hits > 100
