@@ -62,6 +62,15 @@ struct DNframeRandom {
     float shadowDir[DN_MAX_SAMPLES][3];                   // rand_unit_sphere(time*(i+1+numSamples))
 };
 
+// What a voxel's direct rays add per diffuse sample (term.xyz), whether that
+// value holds (term.w == 0: every ray started and ended with lastVoxID = 255),
+// and the ray counters the rays produced (rays, chunk-grid trips, voxel trips,
+// hits), which the pass adds each time it uses the value.
+struct DNdirectTerm {
+    float4 term;
+    uint4 counts;
+};
+
 struct DNlightParams {
     DNsceneView scene;
     const uint32_t* requests;
@@ -93,6 +102,11 @@ struct DNlightParams {
     float firstShadow[6];                 // sunDir + eps (first sample)
     uint32_t tiles;                       // map tiles (bitmap words = ceil(tiles / 32))
     uint32_t copyBack;                    // voxelsOut != voxels: copy lit voxels back after the pass
+    // Direct-ray terms traced ahead of the pass (k_direct), one per request
+    // slot. Null: the pass traces them itself.
+    DNdirectTerm* directOut;              // written by k_direct
+    const DNdirectTerm* directIn;         // read by the pass
+    uint32_t directSlots;                 // capacity of both in request slots
 };
 
 // Batched lighting over many volumes in one persistent launch. Work units are
@@ -262,6 +276,9 @@ bool prepare_lighting(DNvolume* vol, int numDiffuseSamples, int maxDiffuseSample
 // hipError_t value.
 int dn_launch_draw(const DNdrawParams& p, hipStream_t stream);
 int dn_launch_lighting(const DNlightParams& p, uint32_t maxRequests, hipStream_t stream);
+// Direct rays of every voxel of the pass ahead of it, a wave holding 64 / N
+// voxels x N directions (N = numDirectRays), into p.directOut.
+int dn_launch_direct(const DNlightParams& p, uint32_t maxRequests, hipStream_t stream);
 int dn_launch_lighting_epilogue(const DNlightParams& p, uint32_t tiles, const uint32_t* requests,
                                 const uint32_t* numRequests, int copyBack, hipStream_t stream);
 int dn_launch_visible(const DNvisibleParams& p, hipStream_t stream);

@@ -43,7 +43,18 @@ struct LCtx {
     bool firstSample;
     bool dry;                    // compute only: no stores, no bitmap updates
     f3 sunDir, sunStrength;
+    const DNdirectTerm* direct;  // this voxel's direct-ray term traced ahead, or null
 };
+
+// Start point of the lighting rays of stored voxel `sel` of a chunk
+// (voxelLighting.comp:231-232): the voxel centre moved 1/16 - EPS along its normal.
+__device__ __forceinline__ f3 voxel_origin(const DNchunkRec* rec, int32_t sel, f3 normal)
+{
+    const int32_t cx = sel % 8, cy = (sel / 8) % 8, cz = sel / 64;
+    f3 p = (mk((float)cx, (float)cy, (float)cz) * 0.125f + mk((float)rec->pos[0], (float)rec->pos[1], (float)rec->pos[2])) +
+           splat(0.0625f);
+    return p + splat(0.0625f - DN_EPS) * normal;
+}
 
 __device__ void shadow_ray(LCtx& L, f3 p, uint32_t sample, f3& color)
 {
@@ -58,6 +69,19 @@ __device__ void shadow_ray(LCtx& L, f3 p, uint32_t sample, f3& color)
         if(!light_trace(*L.s, *L.c, ld3(r), ld3(r + 3), pp, hv, ca, cm))
             color = color + (L.sunStrength * cm + ca);
         return;
+    }
+    // the terms traced ahead assume every ray starts with lastVoxID = 255 and
+    // ends so (w = 0 when they all did)
+    if(L.direct && L.c->lastVoxID == 255u)
+    {
+        const float4 d = L.direct->term;
+        if(d.w == 0.0f)
+        {
+            const uint4 n = L.direct->counts;
+            L.c->cRays += n.x; L.c->cMap += n.y; L.c->cVox += n.z; L.c->cHits += n.w;
+            color = color + mk(d.x, d.y, d.z);
+            return;
+        }
     }
     f3 acc = splat(0.0f);
     for(uint32_t k = 0; k < P.numDirectRays; k++)
@@ -204,7 +228,6 @@ __device__ void light_voxel(const DNlightParams& P, const DNsceneView& s, uint32
     if(voxNum == 0 && !dry)
         atomicOr(&P.litBits[mapIndex >> 5], 1u << (mapIndex & 31u));
     c.lastVoxID = 255u; c.lastVoxRefract = 1.0f;
-    const int32_t cx = sel % 8, cy = (sel / 8) % 8, cz = sel / 64;
     const uint32_t vi = rec->voxelIndex + voxNum;
     const uint4 cw = s.voxels[vi];
     const DVoxel me = decompress(cw);
@@ -216,9 +239,8 @@ __device__ void light_voxel(const DNlightParams& P, const DNsceneView& s, uint32
     L.firstSample = indirect == 0.0f;
     L.sunDir = ld3(P.sunDir);
     L.sunStrength = ld3(s.sunStrength);
-    f3 p = (mk((float)cx, (float)cy, (float)cz) * 0.125f + mk((float)rec->pos[0], (float)rec->pos[1], (float)rec->pos[2])) +
-           splat(0.0625f);
-    p = p + splat(0.0625f - DN_EPS) * me.normal;
+    L.direct = (P.directIn && P.numDirectRays && slot < P.directSlots) ? P.directIn + slot : nullptr;
+    f3 p = voxel_origin(rec, sel, me.normal);
     f3 spec = splat(0.0f), diff = splat(0.0f);
     f3 view = p - ld3(P.camPos);
     if(mat.specular > 0.0f && dot3(view, me.normal) < 0.0f && mat.reflectType <= 1u)

@@ -135,7 +135,10 @@ struct DNgpuVolume {
     uint32_t* dRequests = nullptr;
     uint32_t* dNumRequests = nullptr;
     uint32_t* dBlockSums = nullptr;
-    uint32_t* dWorkCounter = nullptr;  // [0] lighting requests taken, [1] collected requests, [2..3] draw quads
+    uint32_t* dWorkCounter = nullptr;  // [0] lighting requests taken, [1] collected requests, [2..3] draw quads,
+                                       // [4] direct-ray requests taken
+    DNdirectTerm* dDirect = nullptr;   // direct-ray terms per request slot (k_direct)
+    size_t directCap = 0;              // in slots
     uint32_t* dReqList = nullptr;      // tiles requested by rays (collect)
     uint64_t* dCounters = nullptr;     // 3 kinds x CNT_SLOTS
     float* dPoints = nullptr;          // 2 x DN_MAX_POINTS x 3
@@ -675,7 +678,7 @@ extern "C" DNvolume* DN_create_volume(DNuvec3 mapSize, unsigned int minChunks)
     const uint32_t tiles = mapSize.x * mapSize.y * mapSize.z;
     if(!HIP_OK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "stream") ||
        !alloc_tile_buffers(g, tiles) || !dalloc(&g->dMats, DN_MAX_MATERIALS, "material buffer") ||
-       !dalloc(&g->dNumRequests, 1, "request count") || !dalloc(&g->dWorkCounter, 4, "work counter") ||
+       !dalloc(&g->dNumRequests, 1, "request count") || !dalloc(&g->dWorkCounter, 6, "work counter") ||
        !dalloc(&g->dCounters, 3 * CNT_SLOTS, "counters") || !dalloc(&g->dPoints, 2 * DN_MAX_POINTS * 3, "ray table") ||
        !HIP_OK(hipHostMalloc((void**)&g->hSmall, 64, hipHostMallocDefault), "pinned readback") ||
        !HIP_OK(hipEventCreate(&g->stageDone[0]), "event") || !HIP_OK(hipEventCreate(&g->stageDone[1]), "event"))
@@ -684,7 +687,7 @@ extern "C" DNvolume* DN_create_volume(DNuvec3 mapSize, unsigned int minChunks)
         return nullptr;
     }
     hipMemsetAsync(g->dNumRequests, 0, sizeof(uint32_t), g->stream);
-    hipMemsetAsync(g->dWorkCounter, 0, 4 * sizeof(uint32_t), g->stream);
+    hipMemsetAsync(g->dWorkCounter, 0, 6 * sizeof(uint32_t), g->stream);
     hipMemsetAsync(g->dCounters, 0, 3 * CNT_SLOTS * sizeof(uint64_t), g->stream);
     hipMemsetAsync(g->dMats, 0, sizeof(DNmat) * DN_MAX_MATERIALS, g->stream);
 
@@ -746,7 +749,7 @@ extern "C" void DN_delete_volume(DNvolume* vol)
         if(g->stream) hipStreamSynchronize(g->stream);
         free_tile_buffers(g);
         dfree(g->dVoxels); dfree(g->dVoxelsOut); dfree(g->dMats); dfree(g->dNumRequests); dfree(g->dWorkCounter);
-        dfree(g->dCounters); dfree(g->dPoints); dfree(g->dStage); dfree(g->dEdit); dfree(g->dCube);
+        dfree(g->dCounters); dfree(g->dPoints); dfree(g->dStage); dfree(g->dEdit); dfree(g->dCube); dfree(g->dDirect);
         if(g->hEdit) hipHostFree(g->hEdit);
         for(int s = 0; s < 2; s++)
         {
@@ -1191,6 +1194,8 @@ bool prepare_lighting(DNvolume* vol, int numDiffuseSamples, int maxDiffuseSample
     memset(&p, 0, sizeof(p));
     p.tiles = g->tiles;
     p.copyBack = g->anySpecular ? 1u : 0u;
+    p.directOut = nullptr;
+    p.directIn = nullptr;
     p.scene = scene_view(vol);
     p.requests = g->dRequests;
     p.numRequests = g->dNumRequests;
@@ -1214,8 +1219,9 @@ bool prepare_lighting(DNvolume* vol, int numDiffuseSamples, int maxDiffuseSample
     p.ambient[2] = vol->ambientLightStrength.z;
     p.shadowSoftness = vol->shadowSoftness;
     p.camPos[0] = vol->camPos.x; p.camPos[1] = vol->camPos.y; p.camPos[2] = vol->camPos.z;
-    p.numSpecRays = vol->numSpecularRays;
-    p.numDirectRays = vol->numDirectRays;
+    // the ray tables hold DN_MAX_POINTS directions each
+    p.numSpecRays = std::min<uint32_t>(vol->numSpecularRays, DN_MAX_POINTS);
+    p.numDirectRays = std::min<uint32_t>(vol->numDirectRays, DN_MAX_POINTS);
     p.specPoints = g->dPoints;
     frame_random(vol->lastTime, (uint32_t)numDiffuseSamples, vol->diffuseBounceLimit, p.rnd);
     // voxel-independent shadow rays, in the shader's FP32 operations (voxelLighting.comp:72-78)
@@ -1245,8 +1251,27 @@ extern "C" void DN_update_lighting(DNvolume* vol, int numDiffuseSamples, int max
     DNlightParams p;
     uint32_t maxReq = 0;
     if(!prepare_lighting(vol, numDiffuseSamples, maxDiffuseSamples, time, p, maxReq)) return;
+    // Direct rays traced ahead in direction-major waves (k_direct); voxels whose
+    // rays depend on ray order (transparent voxels) are traced by the pass.
+    bool ahead = p.numDirectRays > 0 && p.numDiffuseSamples > 0 && maxReq > 0;
+    if(ahead && g->directCap < (size_t)maxReq * 32u)
+    {
+        dfree(g->dDirect);
+        g->directCap = 0;
+        if(dalloc(&g->dDirect, (size_t)maxReq * 32u, "direct-ray terms")) g->directCap = (size_t)maxReq * 32u;
+    }
+    ahead = ahead && g->dDirect;
     Timer tm = timer_begin(g, TK_LIGHT);
-    int e = dn_launch_lighting(p, maxReq, g->stream);
+    int e = 0;
+    if(ahead)
+    {
+        p.directOut = g->dDirect;
+        p.directSlots = (uint32_t)std::min<size_t>(g->directCap, 0xFFFFFFFFu);
+        e = dn_launch_direct(p, maxReq, g->stream);
+        p.directOut = nullptr;
+        p.directIn = g->dDirect;
+    }
+    if(!e) e = dn_launch_lighting(p, maxReq, g->stream);
     timer_end(g, tm);
     Timer te = timer_begin(g, TK_EPILOGUE);
     int e2 = dn_launch_lighting_epilogue(p, g->tiles, g->dRequests, g->dNumRequests, g->anySpecular ? 1 : 0, g->stream);

@@ -215,6 +215,116 @@ void k_lighting(DNlightParams P)
     }
 }
 
+// Direct rays of the pass's voxels, traced ahead of k_lighting. The N direct
+// directions are the same for every voxel and lie in a narrow cone around the
+// sun, and a voxel's N rays share their start point: their lengths are alike.
+// So a wave holds 64 / N voxels x N directions (lane = voxel * N + direction)
+// instead of 64 voxels x one direction, and the longest ray of a wave is
+// rarely much longer than the rest. Each voxel's terms are then summed in
+// direction order on the lane of its first direction, exactly as shadow_ray
+// sums them, and stored per request slot (w = 0). Every ray here starts with
+// the shader's lastVoxID = 255; if one of a voxel's rays ends with another
+// value (it stopped inside a transparent voxel), the next ray of the
+// sequential loop would have started differently, so the slot is marked
+// (w = 1) and the pass traces that voxel's rays itself. The rays' counters go
+// with the term (the pass adds them whenever it uses it), so the pass's counts
+// are those of tracing in sequence. Voxels without a diffuse term (specular
+// >= 1) are skipped. A wave takes one request (32 slots) at a time.
+// Waves per SIMD the direct-ray kernel is compiled for: 80 VGPRs, a few
+// spills outside the walks. Measured on the full-frame workload: 4 waves
+// 6.55 ms, 5 waves 5.88, 6 waves 5.49, 7 waves (spilling in the walks) 5.93.
+// The chunk-grid occupancy bitmap is read from L2 here: staging it in LDS
+// (16 KiB per block on that map) would hold the kernel to 4 waves.
+#ifndef DN_DIRECT_WAVES
+#define DN_DIRECT_WAVES 6
+#endif
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DN_DIRECT_WAVES, DN_DIRECT_WAVES)))
+void k_direct(DNlightParams P)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    // per lane: the chunk-mask row of its walks, then (between walks) its
+    // ray's term and counters for the summing lane of its voxel
+    __shared__ uint32_t maskRows[DN_MASK_LDS_WORDS(256)];
+    const DNsceneView sv = P.scene;
+    const uint32_t N = P.numDirectRays;
+    RayCtx c;
+    ctx_init(c);
+    uint32_t* row = maskRows + threadIdx.x * DN_MASK_ROW;
+    c.maskRow = row;
+    const uint32_t nreq = *P.numRequests;
+    const uint32_t per = 64u / N, v = lane / N, k = lane - v * N;
+    const f3 dir = ld3(P.directRays[k]), inv = ld3(P.directRays[k] + 3), sun = ld3(sv.sunStrength);
+    uint32_t* voxRows = maskRows + (threadIdx.x - k) * DN_MASK_ROW;   // row of this voxel's first direction
+    for(;;)
+    {
+        uint32_t r = 0;
+        if(lane == 0) r = atomicAdd(P.workCounter + 4, 1u);
+        r = __shfl(r, 0, 64);
+        if(r >= nreq) break;
+        const uint32_t req = P.requests[r];
+        const DNchunkRec* rec = sv.chunks + (req >> 4);
+        for(uint32_t g = 0; g < 32u; g += per)
+        {
+            const uint32_t lane32 = g + v;
+            const uint32_t voxNum = lane32 + (req & 15u) * 32u;
+            int32_t sel = (v < per && lane32 < 32u) ? select_in(rec->mask, voxNum) : -1;
+            float4 term = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const uint32_t r0 = c.cRays, m0 = c.cMap, v0 = c.cVox, h0 = c.cHits;
+            DVoxel me = {};
+            if(sel >= 0)
+            {
+                me = decompress(sv.voxels[rec->voxelIndex + voxNum]);
+                if(!(sv.mats[me.material].specular < 1.0f)) sel = -1;     // no diffuse term
+            }
+            if(sel >= 0)
+            {
+                f3 pp = voxel_origin(rec, sel, me.normal);
+                c.lastVoxID = 255u; c.lastVoxRefract = 1.0f;
+                DVoxel hv = {};
+                f3 ca;
+                float cm;
+                if(!light_trace_uniform(sv, c, dir, inv, pp, hv, ca, cm))
+                {
+                    const f3 t = sun * cm + ca;
+                    term = make_float4(t.x, t.y, t.z, 1.0f);
+                }
+                if(c.lastVoxID != 255u) term.w += 2.0f;    // ended inside a transparent voxel
+            }
+            __builtin_amdgcn_wave_barrier();      // every lane's walk is done with its row
+            row[0] = __float_as_uint(term.x); row[1] = __float_as_uint(term.y);
+            row[2] = __float_as_uint(term.z); row[3] = __float_as_uint(term.w);
+            row[4] = c.cRays - r0; row[5] = c.cMap - m0; row[6] = c.cVox - v0; row[7] = c.cHits - h0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if(sel >= 0 && k == 0)
+            {
+                f3 acc = splat(0.0f);
+                bool stale = false;
+                uint4 n = make_uint4(0u, 0u, 0u, 0u);
+                for(uint32_t j = 0; j < N; j++)
+                {
+                    const uint32_t* q0 = voxRows + j * DN_MASK_ROW;
+                    const float4 q = make_float4(__uint_as_float(q0[0]), __uint_as_float(q0[1]), __uint_as_float(q0[2]),
+                                                 __uint_as_float(q0[3]));
+                    const uint4 qc = make_uint4(q0[4], q0[5], q0[6], q0[7]);
+                    if(q.w == 1.0f || q.w == 3.0f) acc = acc + mk(q.x, q.y, q.z);
+                    stale = stale || q.w >= 2.0f;
+                    n.x += qc.x; n.y += qc.y; n.z += qc.z; n.w += qc.w;
+                }
+                acc = acc / (float)N;
+                const size_t slot = (size_t)r * 32u + lane32;
+                if(slot < P.directSlots)
+                {
+                    P.directOut[slot].term = make_float4(acc.x, acc.y, acc.z, stale ? 1.0f : 0.0f);
+                    P.directOut[slot].counts = n;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+}
+
 // Pass epilogue over one 32-tile word of the bitmaps: clear the visible bit and
 // bump the sample count of lit chunks, then set the visible bit of chunks
 // reached by specular rays from visible chunks.
@@ -243,7 +353,7 @@ __global__ __launch_bounds__(256) void k_light_epilogue(uint32_t* flags, DNchunk
                                                         uint32_t* visProp, uint32_t words, uint32_t* workCounter)
 {
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    if(w == 0) *workCounter = 0u;
+    if(w == 0) { workCounter[0] = 0u; workCounter[4] = 0u; }
     if(w < words) epilogue_word(flags, chunks, litBits, visProp, w);
 }
 
@@ -596,6 +706,21 @@ int dn_launch_draw(const DNdrawParams& p, hipStream_t stream)
         hipLaunchKernelGGL(k_draw<true>, dim3(blocks), dim3(256), occBytes, stream, p);
     else
         hipLaunchKernelGGL(k_draw<false>, dim3(blocks), dim3(256), 0, stream, p);
+    return (int)hipGetLastError();
+}
+
+int dn_launch_direct(const DNlightParams& p, uint32_t maxRequests, hipStream_t stream)
+{
+    if(p.numDirectRays == 0 || p.numDirectRays > 64u || !p.directOut) return (int)hipErrorInvalidValue;
+    static int perCU = 0;
+    if(perCU == 0)
+    {
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_direct, 256, 0);
+        if(perCU < 1) perCU = 1;
+    }
+    uint32_t blocks = (uint32_t)cu_count() * (uint32_t)perCU;
+    if(maxRequests / 4 + 1 < blocks) blocks = maxRequests / 4 + 1;
+    hipLaunchKernelGGL(k_direct, dim3(blocks), dim3(256), 0, stream, p);
     return (int)hipGetLastError();
 }
 

@@ -471,6 +471,7 @@ __device__ __forceinline__ void site_ctx(const DNlightParams& P, const DNsceneVi
     L.P = &P; L.s = &s; L.c = &c; L.dry = true; L.firstSample = false;
     L.sunDir = ld3(P.sunDir);
     L.sunStrength = ld3(s.sunStrength);
+    L.direct = nullptr;
 }
 
 __global__ __launch_bounds__(256) void k_rc_samples(DNlightParams P, uint32_t n, uint32_t seed, float* samples)
