@@ -127,3 +127,19 @@ def test_demo_scene_lighting_parity(gpu):
             _, n = run_frame_and_compare(vol, sc, o, f, 50.0 + f / 60.0, sc.width, sc.height)
             lit += n
         assert lit > 2000
+
+
+def test_demo_scene_direct_rays_through_glass(gpu):
+    """The demo scene with 16 direct rays: its glass (opacity 0.5) makes some
+    rays stop inside a transparent voxel, where the shader carries lastVoxID
+    from one ray to the next; those voxels must fall back from the direct terms
+    traced ahead to tracing in sequence, and every voxel must still match."""
+    o = oracle.load("f32")
+    with Volume.load(DEMO) as vol:
+        vol.v.numDirectRays = 16
+        sc = demo_scene(vol)
+        vol.request_all()
+        vol.sync()
+        vol.synchronize()
+        for f in range(2):
+            run_frame_and_compare(vol, sc, o, f, 60.0 + f / 60.0, sc.width, sc.height)
