@@ -121,6 +121,7 @@ struct DNbatchParams {
     uint32_t queues;               // 1 or 8
     uint32_t dry;                  // 1: compute only, no stores (autotuning)
     uint32_t* queueCounters;       // [queues] next unit of each queue
+    uint32_t* directCounters;      // [queues] next unit of each queue for the direct-ray pass
     const uint32_t* queueScenes;   // [queues][numScenes] volume ids of each queue, in order
     const uint32_t* queueLen;      // [queues] volumes in each queue
     uint32_t* queuePrefix;         // [queues][numScenes + 1] exclusive unit prefix, written on device
@@ -289,8 +290,9 @@ int dn_launch_rays(const DNrayParams& p, hipStream_t stream);
 int dn_launch_collect_requested(const uint32_t* flags, uint32_t tiles, uint32_t* list, uint32_t* count,
                                 hipStream_t stream);
 int dn_launch_mark_visible(uint32_t* flags, const uint32_t* list, uint32_t n, hipStream_t stream);
-// Batched pass: unit prefix + counter reset, persistent lighting kernel, then
-// the per-volume epilogue and copy-back. `blockThreads` 64 or 256, `blocksPerCU`
+// Batched pass: unit prefix + counter reset, with `direct` the direct rays of
+// the volumes whose parameters carry directOut (k_direct_batch), the persistent
+// lighting kernel, then the per-volume epilogue and copy-back. `blockThreads` 64 or 256, `blocksPerCU`
 // 0 = as many as fit. `maxTiles` / `maxRequests`: largest over the volumes.
 int dn_launch_batch(const DNbatchParams& b, uint32_t blockThreads, uint32_t blocksPerCU, uint32_t maxTiles,
-                    uint32_t maxRequests, bool epilogue, hipStream_t stream);
+                    uint32_t maxRequests, bool epilogue, bool direct, hipStream_t stream);

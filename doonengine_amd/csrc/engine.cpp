@@ -579,6 +579,21 @@ DNsceneView scene_view(DNvolume* vol)
     return s;
 }
 
+// Direct-ray term buffer (k_direct) for up to maxReq requests. Growing frees
+// the old buffer after the device is idle (a batch graph on another stream
+// may still read it).
+bool ensure_direct(DNgpuVolume* g, uint32_t maxReq)
+{
+    const size_t need = (size_t)maxReq * 32u;
+    if(g->dDirect && g->directCap >= need) return true;
+    hipDeviceSynchronize();
+    dfree(g->dDirect);
+    g->directCap = 0;
+    if(!dalloc(&g->dDirect, need, "direct-ray terms")) return false;
+    g->directCap = need;
+    return true;
+}
+
 void clear_chunk(DNvolume* vol, size_t index)   // voxel.c:1353-1363
 {
     DNchunk& c = vol->chunks[index];
@@ -1253,14 +1268,7 @@ extern "C" void DN_update_lighting(DNvolume* vol, int numDiffuseSamples, int max
     if(!prepare_lighting(vol, numDiffuseSamples, maxDiffuseSamples, time, p, maxReq)) return;
     // Direct rays traced ahead in direction-major waves (k_direct); voxels whose
     // rays depend on ray order (transparent voxels) are traced by the pass.
-    bool ahead = p.numDirectRays > 0 && p.numDiffuseSamples > 0 && maxReq > 0;
-    if(ahead && g->directCap < (size_t)maxReq * 32u)
-    {
-        dfree(g->dDirect);
-        g->directCap = 0;
-        if(dalloc(&g->dDirect, (size_t)maxReq * 32u, "direct-ray terms")) g->directCap = (size_t)maxReq * 32u;
-    }
-    ahead = ahead && g->dDirect;
+    const bool ahead = p.numDirectRays > 0 && p.numDiffuseSamples > 0 && maxReq > 0 && ensure_direct(g, maxReq);
     Timer tm = timer_begin(g, TK_LIGHT);
     int e = 0;
     if(ahead)
@@ -2422,6 +2430,8 @@ struct DNXbatch {
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     DNXbatchConfig execCfg[2];
     uint32_t execTiles[2] = {0, 0}, execReq[2] = {0, 0};
+    bool execDirect[2] = {false, false};
+    bool anyDirect = false;                           // some volume traces its direct rays ahead this frame
     bool timing = true;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, spare;
     double ms = 0.0;
@@ -2468,6 +2478,7 @@ DNbatchParams batch_params(DNXbatch* b, bool dry)
     bp.queues = b->queues;
     bp.dry = dry ? 1u : 0u;
     bp.queueCounters = b->dQueueCounters;
+    bp.directCounters = b->dQueueCounters + DN_BATCH_MAX_QUEUES;
     bp.queueScenes = b->dQueueScenes;
     bp.queueLen = b->dQueueLen;
     bp.queuePrefix = b->dQueuePrefix;
@@ -2481,11 +2492,20 @@ bool batch_prepare(DNXbatch* b, DNlightParams* dst, int nds, int mds, float time
 {
     maxTiles = 0;
     maxReq = 0;
+    b->anyDirect = false;
     for(size_t i = 0; i < b->vols.size(); i++)
     {
         DNvolume* vol = b->vols[i];
         uint32_t mr = 0;
         if(!prepare_lighting(vol, nds, mds, time, dst[i], mr)) return false;
+        DNgpuVolume* g = vol->gpu;
+        if(dst[i].numDirectRays > 0 && dst[i].numDiffuseSamples > 0 && mr > 0 && ensure_direct(g, mr))
+        {
+            dst[i].directOut = g->dDirect;
+            dst[i].directIn = g->dDirect;
+            dst[i].directSlots = (uint32_t)std::min<size_t>(g->directCap, 0xFFFFFFFFu);
+            b->anyDirect = true;
+        }
         maxTiles = std::max(maxTiles, vol->gpu->tiles);
         maxReq = std::max(maxReq, mr);
         hipEventRecord(b->ready, vol->gpu->stream);
@@ -2528,7 +2548,7 @@ extern "C" DNXbatch* DNX_batch_create(DNvolume* const* volumes, uint32_t count)
                     "batch parameters") &&
              HIP_OK(hipEventCreateWithFlags(&b->slotDone[s], hipEventDisableTiming), "batch event");
     ok = ok && dalloc(&b->dParams, count, "batch parameters") &&
-         dalloc(&b->dQueueCounters, DN_BATCH_MAX_QUEUES, "batch queues") &&
+         dalloc(&b->dQueueCounters, 2 * DN_BATCH_MAX_QUEUES, "batch queues") &&
          dalloc(&b->dQueueScenes, (size_t)DN_BATCH_MAX_QUEUES * count, "batch queues") &&
          dalloc(&b->dQueueLen, DN_BATCH_MAX_QUEUES, "batch queues") &&
          dalloc(&b->dQueuePrefix, (size_t)DN_BATCH_MAX_QUEUES * (count + 1u), "batch queues");
@@ -2588,7 +2608,7 @@ extern "C" bool DNX_batch_update_lighting(DNXbatch* b, int numDiffuseSamples, in
     const DNXbatchConfig& e = b->execCfg[s];
     bool stale = !b->exec[s] || e.requestsPerUnit != c.requestsPerUnit || e.blockThreads != c.blockThreads ||
                  e.blocksPerCU != c.blocksPerCU || e.xcdQueues != c.xcdQueues || b->execTiles[s] != maxTiles ||
-                 b->execReq[s] != maxReq;
+                 b->execReq[s] != maxReq || b->execDirect[s] != b->anyDirect;
     if(stale)
     {
         if(b->exec[s]) { hipGraphExecDestroy(b->exec[s]); b->exec[s] = nullptr; }
@@ -2601,7 +2621,8 @@ extern "C" bool DNX_batch_update_lighting(DNXbatch* b, int numDiffuseSamples, in
         {
             hipMemcpyAsync(b->dParams, b->hParams[s], sizeof(DNlightParams) * b->vols.size(), hipMemcpyHostToDevice,
                            cs);
-            int le = dn_launch_batch(batch_params(b, false), c.blockThreads, c.blocksPerCU, maxTiles, maxReq, true, cs);
+            int le = dn_launch_batch(batch_params(b, false), c.blockThreads, c.blocksPerCU, maxTiles, maxReq, true,
+                                     b->anyDirect, cs);
             ok = HIP_OK(hipStreamEndCapture(cs, &graph), "batch capture") && le == 0;
             if(ok) ok = HIP_OK(hipGraphInstantiate(&b->exec[s], graph, nullptr, nullptr, 0), "batch graph");
             if(graph) hipGraphDestroy(graph);
@@ -2611,6 +2632,7 @@ extern "C" bool DNX_batch_update_lighting(DNXbatch* b, int numDiffuseSamples, in
         b->execCfg[s] = c;
         b->execTiles[s] = maxTiles;
         b->execReq[s] = maxReq;
+        b->execDirect[s] = b->anyDirect;
     }
     std::pair<hipEvent_t, hipEvent_t> t(nullptr, nullptr);
     if(b->timing)
@@ -2683,7 +2705,7 @@ extern "C" bool DNX_batch_autotune(DNXbatch* b, int numDiffuseSamples, int maxDi
                 for(int r = 0; r < repeats; r++)
                 {
                     hipEventRecord(e0, b->stream);
-                    dn_launch_batch(batch_params(b, true), th, 0, maxTiles, maxReq, false, b->stream);
+                    dn_launch_batch(batch_params(b, true), th, 0, maxTiles, maxReq, false, b->anyDirect, b->stream);
                     hipEventRecord(e1, b->stream);
                     hipEventSynchronize(e1);
                     hipEventElapsedTime(&ms, e0, e1);

@@ -238,6 +238,76 @@ void k_lighting(DNlightParams P)
 #ifndef DN_DIRECT_WAVES
 #define DN_DIRECT_WAVES 6
 #endif
+// The direct rays of one request's 32 slots: lane = voxel * N + direction,
+// 64 / N voxels at a time. `row` is the lane's mask row, `voxRows` the row of
+// its voxel's first direction.
+__device__ __forceinline__ void direct_request(const DNlightParams& P, const DNsceneView& sv, uint32_t r, RayCtx& c,
+                                               uint32_t* row, const uint32_t* voxRows, uint32_t N, uint32_t per,
+                                               uint32_t v, uint32_t k, f3 dir, f3 inv, f3 sun)
+{
+    const uint32_t req = P.requests[r];
+    const DNchunkRec* rec = sv.chunks + (req >> 4);
+    for(uint32_t g = 0; g < 32u; g += per)
+    {
+        const uint32_t lane32 = g + v;
+        const uint32_t voxNum = lane32 + (req & 15u) * 32u;
+        int32_t sel = (v < per && lane32 < 32u) ? select_in(rec->mask, voxNum) : -1;
+        float4 term = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const uint32_t r0 = c.cRays, m0 = c.cMap, v0 = c.cVox, h0 = c.cHits;
+        DVoxel me = {};
+        if(sel >= 0)
+        {
+            me = decompress(sv.voxels[rec->voxelIndex + voxNum]);
+            if(!(sv.mats[me.material].specular < 1.0f)) sel = -1;     // no diffuse term
+        }
+        if(sel >= 0)
+        {
+            f3 pp = voxel_origin(rec, sel, me.normal);
+            c.lastVoxID = 255u; c.lastVoxRefract = 1.0f;
+            DVoxel hv = {};
+            f3 ca;
+            float cm;
+            if(!light_trace_uniform(sv, c, dir, inv, pp, hv, ca, cm))
+            {
+                const f3 t = sun * cm + ca;
+                term = make_float4(t.x, t.y, t.z, 1.0f);
+            }
+            if(c.lastVoxID != 255u) term.w += 2.0f;    // ended inside a transparent voxel
+        }
+        __builtin_amdgcn_wave_barrier();      // every lane's walk is done with its row
+        row[0] = __float_as_uint(term.x); row[1] = __float_as_uint(term.y);
+        row[2] = __float_as_uint(term.z); row[3] = __float_as_uint(term.w);
+        row[4] = c.cRays - r0; row[5] = c.cMap - m0; row[6] = c.cVox - v0; row[7] = c.cHits - h0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if(sel >= 0 && k == 0)
+        {
+            f3 acc = splat(0.0f);
+            bool stale = false;
+            uint4 n = make_uint4(0u, 0u, 0u, 0u);
+            for(uint32_t j = 0; j < N; j++)
+            {
+                const uint32_t* q0 = voxRows + j * DN_MASK_ROW;
+                const float4 q = make_float4(__uint_as_float(q0[0]), __uint_as_float(q0[1]), __uint_as_float(q0[2]),
+                                             __uint_as_float(q0[3]));
+                const uint4 qc = make_uint4(q0[4], q0[5], q0[6], q0[7]);
+                if(q.w == 1.0f || q.w == 3.0f) acc = acc + mk(q.x, q.y, q.z);
+                stale = stale || q.w >= 2.0f;
+                n.x += qc.x; n.y += qc.y; n.z += qc.z; n.w += qc.w;
+            }
+            acc = acc / (float)N;
+            const size_t slot = (size_t)r * 32u + lane32;
+            if(slot < P.directSlots)
+            {
+                P.directOut[slot].term = make_float4(acc.x, acc.y, acc.z, stale ? 1.0f : 0.0f);
+                P.directOut[slot].counts = n;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DN_DIRECT_WAVES, DN_DIRECT_WAVES)))
 void k_direct(DNlightParams P)
 {
@@ -254,73 +324,62 @@ void k_direct(DNlightParams P)
     const uint32_t nreq = *P.numRequests;
     const uint32_t per = 64u / N, v = lane / N, k = lane - v * N;
     const f3 dir = ld3(P.directRays[k]), inv = ld3(P.directRays[k] + 3), sun = ld3(sv.sunStrength);
-    uint32_t* voxRows = maskRows + (threadIdx.x - k) * DN_MASK_ROW;   // row of this voxel's first direction
+    const uint32_t* voxRows = maskRows + (threadIdx.x - k) * DN_MASK_ROW;   // row of this voxel's first direction
     for(;;)
     {
         uint32_t r = 0;
         if(lane == 0) r = atomicAdd(P.workCounter + 4, 1u);
         r = __shfl(r, 0, 64);
         if(r >= nreq) break;
-        const uint32_t req = P.requests[r];
-        const DNchunkRec* rec = sv.chunks + (req >> 4);
-        for(uint32_t g = 0; g < 32u; g += per)
+        direct_request(P, sv, r, c, row, voxRows, N, per, v, k, dir, inv, sun);
+    }
+}
+
+// k_direct over the volumes of a batch, with the work units, queues and
+// per-queue counters of k_lighting_batch (its own counters). Volumes whose pass
+// traces its direct rays itself (no directOut) are skipped.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DN_DIRECT_WAVES, DN_DIRECT_WAVES)))
+void k_direct_batch(DNbatchParams B, const DNlightParams* __restrict__ scenes)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t home = blockIdx.x % B.queues;
+    __shared__ uint32_t maskRows[DN_MASK_LDS_WORDS(256)];
+    RayCtx c;
+    ctx_init(c);
+    uint32_t* row = maskRows + threadIdx.x * DN_MASK_ROW;
+    c.maskRow = row;
+    for(uint32_t qi = 0; qi < B.queues; qi++)
+    {
+        const uint32_t q = (home + qi) % B.queues;
+        const uint32_t len = B.queueLen[q];
+        if(len == 0) continue;
+        const uint32_t* pre = B.queuePrefix + (size_t)q * (B.numScenes + 1u);
+        const uint32_t total = pre[len];
+        for(;;)
         {
-            const uint32_t lane32 = g + v;
-            const uint32_t voxNum = lane32 + (req & 15u) * 32u;
-            int32_t sel = (v < per && lane32 < 32u) ? select_in(rec->mask, voxNum) : -1;
-            float4 term = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            const uint32_t r0 = c.cRays, m0 = c.cMap, v0 = c.cVox, h0 = c.cHits;
-            DVoxel me = {};
-            if(sel >= 0)
+            uint32_t u = 0;
+            if(lane == 0) u = atomicAdd(&B.directCounters[q], 1u);
+            u = __builtin_amdgcn_readfirstlane(__shfl(u, 0, 64));
+            if(u >= total) break;
+            uint32_t lo = 0, hi = len - 1u;           // last volume with pre[i] <= u
+            while(lo < hi)
             {
-                me = decompress(sv.voxels[rec->voxelIndex + voxNum]);
-                if(!(sv.mats[me.material].specular < 1.0f)) sel = -1;     // no diffuse term
+                const uint32_t mid = (lo + hi + 1u) >> 1;
+                if(pre[mid] <= u) lo = mid;
+                else hi = mid - 1u;
             }
-            if(sel >= 0)
-            {
-                f3 pp = voxel_origin(rec, sel, me.normal);
-                c.lastVoxID = 255u; c.lastVoxRefract = 1.0f;
-                DVoxel hv = {};
-                f3 ca;
-                float cm;
-                if(!light_trace_uniform(sv, c, dir, inv, pp, hv, ca, cm))
-                {
-                    const f3 t = sun * cm + ca;
-                    term = make_float4(t.x, t.y, t.z, 1.0f);
-                }
-                if(c.lastVoxID != 255u) term.w += 2.0f;    // ended inside a transparent voxel
-            }
-            __builtin_amdgcn_wave_barrier();      // every lane's walk is done with its row
-            row[0] = __float_as_uint(term.x); row[1] = __float_as_uint(term.y);
-            row[2] = __float_as_uint(term.z); row[3] = __float_as_uint(term.w);
-            row[4] = c.cRays - r0; row[5] = c.cMap - m0; row[6] = c.cVox - v0; row[7] = c.cHits - h0;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if(sel >= 0 && k == 0)
-            {
-                f3 acc = splat(0.0f);
-                bool stale = false;
-                uint4 n = make_uint4(0u, 0u, 0u, 0u);
-                for(uint32_t j = 0; j < N; j++)
-                {
-                    const uint32_t* q0 = voxRows + j * DN_MASK_ROW;
-                    const float4 q = make_float4(__uint_as_float(q0[0]), __uint_as_float(q0[1]), __uint_as_float(q0[2]),
-                                                 __uint_as_float(q0[3]));
-                    const uint4 qc = make_uint4(q0[4], q0[5], q0[6], q0[7]);
-                    if(q.w == 1.0f || q.w == 3.0f) acc = acc + mk(q.x, q.y, q.z);
-                    stale = stale || q.w >= 2.0f;
-                    n.x += qc.x; n.y += qc.y; n.z += qc.z; n.w += qc.w;
-                }
-                acc = acc / (float)N;
-                const size_t slot = (size_t)r * 32u + lane32;
-                if(slot < P.directSlots)
-                {
-                    P.directOut[slot].term = make_float4(acc.x, acc.y, acc.z, stale ? 1.0f : 0.0f);
-                    P.directOut[slot].counts = n;
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
+            const uint32_t sid = __builtin_amdgcn_readfirstlane(B.queueScenes[(size_t)q * B.numScenes + lo]);
+            const DNlightParams& P = scenes[sid];
+            if(!P.directOut) continue;
+            const DNsceneView sv = P.scene;
+            const uint32_t N = P.numDirectRays;
+            const uint32_t per = 64u / N, v = lane / N, k = lane - v * N;
+            const f3 dir = ld3(P.directRays[k]), inv = ld3(P.directRays[k] + 3), sun = ld3(sv.sunStrength);
+            const uint32_t* voxRows = maskRows + (threadIdx.x - k) * DN_MASK_ROW;
+            const uint32_t nreq = *P.numRequests;
+            const uint32_t r0 = (u - pre[lo]) * B.unit;
+            for(uint32_t j = 0; j < B.unit && r0 + j < nreq; j++)
+                direct_request(P, sv, r0 + j, c, row, voxRows, N, per, v, k, dir, inv, sun);
         }
     }
 }
@@ -547,7 +606,7 @@ __global__ __launch_bounds__(1024) void k_batch_prefix(DNbatchParams B)
     __shared__ uint32_t waveSums[16];
     __shared__ uint32_t carry;
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
-    if(t < B.queues) B.queueCounters[t] = 0u;
+    if(t < B.queues) { B.queueCounters[t] = 0u; B.directCounters[t] = 0u; }
     for(uint32_t q = 0; q < B.queues; q++)
     {
         const uint32_t len = B.queueLen[q];
@@ -764,13 +823,24 @@ int dn_launch_lighting_epilogue(const DNlightParams& p, uint32_t tiles, const ui
 }
 
 int dn_launch_batch(const DNbatchParams& b, uint32_t blockThreads, uint32_t blocksPerCU, uint32_t maxTiles,
-                    uint32_t maxRequests, bool epilogue, hipStream_t stream)
+                    uint32_t maxRequests, bool epilogue, bool direct, hipStream_t stream)
 {
     if(b.numScenes == 0) return 0;
     if(b.numScenes > 65535u || b.unit < 2u || (b.unit & 1u) || b.queues < 1u || b.queues > DN_BATCH_MAX_QUEUES ||
        (blockThreads != 64u && blockThreads != 256u))
         return (int)hipErrorInvalidValue;
     hipLaunchKernelGGL(k_batch_prefix, dim3(1), dim3(1024), 0, stream, b);
+    if(direct)
+    {
+        static int perCUd = 0;
+        if(perCUd == 0)
+        {
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCUd, k_direct_batch, 256, 0);
+            if(perCUd < 1) perCUd = 1;
+        }
+        hipLaunchKernelGGL(k_direct_batch, dim3((uint32_t)cu_count() * (uint32_t)perCUd), dim3(256), 0, stream, b,
+                           b.scenes);
+    }
     static int perCU64 = 0, perCU256 = 0;
     int& perCU = blockThreads == 64u ? perCU64 : perCU256;
     if(perCU == 0)
