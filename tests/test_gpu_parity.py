@@ -290,3 +290,40 @@ def test_cubemap_sky_draw_and_lighting_parity(gpu):
     for f in range(2):
         run_frame_and_compare(vol, sc, o, f, 500.0 + f / 60.0, sc.width, sc.height)
     vol.close()
+
+
+@pytest.mark.parametrize("n_direct", [1, 24, 64])
+def test_lighting_parity_direct_ray_counts(gpu, n_direct):
+    """Direct-ray counts that fill a wave unevenly (64 / N voxels x N directions
+    per wave: 64 x 1, 2 x 24 with 16 idle lanes, 1 x 64)."""
+    sc = scenes.config_small(seed=16)
+    o = oracle.load("f32")
+    vol = load_scene(sc)
+    vol.v.numDirectRays = n_direct
+    run_frame_and_compare(vol, sc, o, 0, 600.0, sc.width, sc.height)
+    run_frame_and_compare(vol, sc, o, 1, 600.0 + 1 / 60.0, sc.width, sc.height)
+    vol.close()
+
+
+@pytest.mark.parametrize("size", [(97, 61), (8, 8), (1, 200), (333, 7)])
+def test_draw_parity_odd_image_sizes(gpu, size):
+    """The persistent draw takes 8 x 8 quads: images whose sides are not
+    multiples of 8, a single quad, a one-pixel column."""
+    sc = scenes.config_small(seed=17)
+    o = oracle.load("f32")
+    vol = load_scene(sc)
+    w, h = size
+    s, (m, c, v) = oracle_scene(o, vol, sc)
+    rgba, px, view, proj = draw_gpu(vol, sc, w, h)
+    from doonengine_amd.volume import draw_matrices
+    iv, ivp = draw_matrices(view, proj)
+    flags = m[:, 0].copy()
+    orgba, opx, ocnt = o.draw(s, mat_to_np(iv), mat_to_np(ivp), w, h, flags_out=flags)
+    for f in ("voxelIndex", "mapSteps", "voxelSteps"):
+        assert np.array_equal(px[f], opx[f]), f
+    np.testing.assert_allclose(rgba[..., :3], orgba[..., :3], rtol=2e-6, atol=1e-7)
+    m2, _, _ = vol.export_state()
+    assert np.array_equal(m2[:, 0], flags)
+    cnt = vol.counters(0, reset=True)
+    assert cnt["rays"] == ocnt["calls"] == w * h
+    vol.close()
