@@ -15,7 +15,6 @@
 #include <cstring>
 #include <set>
 #include <thread>
-#include <unordered_map>
 #include <vector>
 
 void (*g_DN_message_callback)(DNmessageType, DNmessageSeverity, const char*) = nullptr;
@@ -161,6 +160,7 @@ struct DNgpuVolume {
     // device sphere edits: device copy of the CPU chunk words (allocated on first use)
     uint2* dEditWords = nullptr;
     std::vector<uint8_t> editValid;    // per tile: device words equal the CPU chunk's
+    std::vector<int32_t> editSlot;     // per tile: its index among a batch's tiles, -1 between calls
     uint8_t* hEdit = nullptr;          // pinned in/out staging
     size_t hEditCap = 0;
     uint8_t* dEdit = nullptr;          // device in/out staging
@@ -261,6 +261,7 @@ void free_tile_buffers(DNgpuVolume* g)
     dfree(g->dFresh); dfree(g->dLitBits); dfree(g->dVisProp); dfree(g->dRequests); dfree(g->dBlockSums);
     dfree(g->dReqList); dfree(g->dMarkList); dfree(g->dEditWords);
     g->editValid.clear();
+    g->editSlot.clear();
     g->markValid = false;
     g->exposedValid = false;
 }
@@ -1917,6 +1918,7 @@ extern "C" bool DNX_shape_spheres(DNvolume* vol, const DNXsphereEdit* edits, uin
     {
         if(!dalloc(&g->dEditWords, (size_t)tiles * 512u, "device copy of the chunk words")) return false;
         g->editValid.assign(tiles, 0);
+        g->editSlot.assign(tiles, -1);
     }
 
     // (tile, sphere) pairs in the order DN_shape_sphere visits them
@@ -1949,15 +1951,23 @@ extern "C" bool DNX_shape_spheres(DNvolume* vol, const DNXsphereEdit* edits, uin
     if(P == 0) return true;
 
     // tiles in first-touch order, each with its pairs in edit order
-    std::unordered_map<uint32_t, uint32_t> slotOf;
-    slotOf.reserve(P);
+    std::vector<int32_t>& slotOf = g->editSlot;
     std::vector<uint32_t> tileIds, pairCount;
     for(const Pair& q : pairs)
     {
-        auto it = slotOf.emplace(q.tile, (uint32_t)tileIds.size());
-        if(it.second) { tileIds.push_back(q.tile); pairCount.push_back(0); }
-        pairCount[it.first->second]++;
+        if(slotOf[q.tile] < 0)
+        {
+            slotOf[q.tile] = (int32_t)tileIds.size();
+            tileIds.push_back(q.tile);
+            pairCount.push_back(0);
+        }
+        pairCount[slotOf[q.tile]]++;
     }
+    struct SlotReset {                  // every exit leaves the slot table clear
+        std::vector<int32_t>& slot;
+        const std::vector<uint32_t>& ids;
+        ~SlotReset() { for(uint32_t t : ids) slot[t] = -1; }
+    } slotReset{slotOf, tileIds};
     const uint32_t U = (uint32_t)tileIds.size();
     std::vector<uint32_t> stale;
     for(uint32_t u = 0; u < U; u++)
