@@ -137,7 +137,6 @@ struct DNdrawParams {
     void* hitOut;                  // optional DNXpixelHit*
     uint64_t* counters;
     uint32_t* quadCounter;         // device [2]: next 8x8 quad, finished waves; zero between launches
-    uint32_t tiles;                // map tiles (occupancy bitmap words = ceil(tiles / 32))
 };
 
 struct DNrayParams {

@@ -1179,7 +1179,6 @@ extern "C" void DN_draw(DNvolume* vol, DNtexture out, DNmat4 view, DNmat4 projec
         dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR, "draw hit output smaller than the image; not written");
     p.counters = g->dCounters + 0 * CNT_SLOTS;
     p.quadCounter = g->dWorkCounter + 2;
-    p.tiles = g->tiles;
     Timer tm = timer_begin(g, TK_DRAW);
     int e = dn_launch_draw(p, g->stream);
     timer_end(g, tm);

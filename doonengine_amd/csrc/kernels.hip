@@ -92,28 +92,20 @@ __device__ __forceinline__ void draw_pixel(const DNdrawParams& P, const DNsceneV
 }
 
 // Persistent draw: each wave takes 8x8 pixel quads (numbered row by row) from
-// a device counter, lane l drawing pixel (l & 7, l >> 3) of the quad. The
-// chunk-grid occupancy bitmap is staged in LDS once per block, and chunk masks
-// go to per-lane LDS rows on chunk entry. The last wave to finish resets the
-// quad counter for the next launch.
+// a device counter, lane l drawing pixel (l & 7, l >> 3) of the quad. Chunk
+// masks go to per-lane LDS rows on chunk entry; the chunk-grid occupancy bits
+// are read through L2 (staging the bitmap in LDS per block measured 0.75 vs
+// 0.71 ms at 1920x1080: neighbouring primary rays read the same words). The
+// last wave to finish resets the quad counter for the next launch.
 #ifndef DN_DRAW_WAVES
 #define DN_DRAW_WAVES 4
 #endif
-template <bool LDS_OCC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DN_DRAW_WAVES, DN_DRAW_WAVES)))
 void k_draw(DNdrawParams P)
 {
     const uint32_t lane = threadIdx.x & 63u;
     __shared__ uint32_t maskRows[DN_MASK_LDS_WORDS(256)];
-    extern __shared__ uint32_t occLds[];
-    DNsceneView sv = P.scene;
-    if(LDS_OCC)
-    {
-        const uint32_t words = (P.tiles + 31u) / 32u;
-        for(uint32_t i = threadIdx.x; i < words; i += 256u) occLds[i] = P.scene.occLoaded[i];
-        __syncthreads();
-        sv.occLoaded = occLds;
-    }
+    const DNsceneView sv = P.scene;
     RayCtx c;
     ctx_init(c);
     c.maskRow = maskRows + threadIdx.x * DN_MASK_ROW;
@@ -746,25 +738,16 @@ int cu_count()
 int dn_launch_draw(const DNdrawParams& p, hipStream_t stream)
 {
     // persistent grid of resident blocks, no more than one wave per quad
-    static int perCU[2] = {0, 0};
-    static size_t lastOccBytes = ~(size_t)0;
-    const size_t occBytes = sizeof(uint32_t) * ((p.tiles + 31u) / 32u);
-    const bool lds = occBytes <= DN_OCC_LDS_MAX;
-    if(perCU[lds] == 0 || (lds && occBytes != lastOccBytes))
+    static int perCU = 0;
+    if(perCU == 0)
     {
-        int n = 0;
-        if(lds) hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_draw<true>, 256, occBytes);
-        else hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_draw<false>, 256, 0);
-        perCU[lds] = n < 1 ? 1 : n;
-        if(lds) lastOccBytes = occBytes;
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_draw, 256, 0);
+        if(perCU < 1) perCU = 1;
     }
     const uint32_t quads = ((p.width + 7u) / 8u) * ((p.height + 7u) / 8u);
-    uint32_t blocks = (uint32_t)cu_count() * (uint32_t)perCU[lds];
+    uint32_t blocks = (uint32_t)cu_count() * (uint32_t)perCU;
     if((quads + 3u) / 4u < blocks) blocks = (quads + 3u) / 4u;
-    if(lds)
-        hipLaunchKernelGGL(k_draw<true>, dim3(blocks), dim3(256), occBytes, stream, p);
-    else
-        hipLaunchKernelGGL(k_draw<false>, dim3(blocks), dim3(256), 0, stream, p);
+    hipLaunchKernelGGL(k_draw, dim3(blocks), dim3(256), 0, stream, p);
     return (int)hipGetLastError();
 }
 
