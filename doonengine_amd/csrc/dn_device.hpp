@@ -111,22 +111,38 @@ __device__ __forceinline__ uint32_t rank_in(const Mask512& m, const uint32_t par
 
 // get_voxel_position (voxelShared.comp:172-224): position index of the
 // (voxNum+1)-th stored voxel, or -1.
+// Position of the k-th (from 0) set bit of m, which has more than k set bits:
+// a binary search over popcounts of halves, quarters, ... of the word.
+__device__ __forceinline__ uint32_t select_bit32(uint32_t m, uint32_t k)
+{
+    uint32_t pos = 0, c;
+    c = __builtin_popcount(m & 0xFFFFu); if(k >= c) { k -= c; m >>= 16; pos += 16; }
+    c = __builtin_popcount(m & 0xFFu);   if(k >= c) { k -= c; m >>= 8; pos += 8; }
+    c = __builtin_popcount(m & 0xFu);    if(k >= c) { k -= c; m >>= 4; pos += 4; }
+    c = __builtin_popcount(m & 0x3u);    if(k >= c) { k -= c; m >>= 2; pos += 2; }
+    if(k >= (m & 1u)) pos += 1;
+    return pos;
+}
+
+// get_voxel_position's select (voxelShared.comp:172-224): the bit index of the
+// voxNum-th (from 0) stored voxel of a chunk, or -1 past the last one. `mask`
+// is a chunk record's 16-word mask (16-byte aligned): its four 16-byte pieces
+// are loaded at once, then the word holding the bit is found by popcounts.
 __device__ __forceinline__ int32_t select_in(const uint32_t* mask, uint32_t voxNum)
 {
+    const uint4* q = reinterpret_cast<const uint4*>(mask);
+    const uint4 a = q[0], b = q[1], c4 = q[2], d = q[3];
+    const uint32_t w[16] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c4.x, c4.y, c4.z, c4.w, d.x, d.y, d.z, d.w};
     uint32_t count = 0;
+    int32_t result = -1;
+    #pragma unroll
     for(uint32_t i = 0; i < 16; i++)
     {
-        uint32_t word = mask[i];
-        uint32_t bc = __builtin_popcount(word);
-        if(count + bc > voxNum)
-        {
-            uint32_t k = voxNum - count;   // k-th set bit of word (0-based)
-            for(uint32_t t = 0; t < k; t++) word &= word - 1u;
-            return (int32_t)(i * 32u + (uint32_t)__builtin_ctz(word));
-        }
+        const uint32_t bc = (uint32_t)__builtin_popcount(w[i]);
+        if(result < 0 && count + bc > voxNum) result = (int32_t)(i * 32u + select_bit32(w[i], voxNum - count));
         count += bc;
     }
-    return -1;
+    return result;
 }
 
 // ---------------------------------------------------------------------------

@@ -239,22 +239,29 @@ __device__ __forceinline__ void direct_request(const DNlightParams& P, const DNs
 {
     const uint32_t req = P.requests[r];
     const DNchunkRec* rec = sv.chunks + (req >> 4);
+    // each slot's start point once per request: lane l prepares slot l & 31,
+    // the tracing lanes fetch theirs by shuffle
+    const uint32_t myNum = (threadIdx.x & 31u) + (req & 15u) * 32u;
+    int32_t selL = select_in(rec->mask, myNum);
+    f3 pL = splat(0.0f);
+    if(selL >= 0)
+    {
+        const DVoxel me = decompress(sv.voxels[rec->voxelIndex + myNum]);
+        if(sv.mats[me.material].specular < 1.0f) pL = voxel_origin(rec, selL, me.normal);
+        else selL = -1;                                              // no diffuse term
+    }
     for(uint32_t g = 0; g < 32u; g += per)
     {
         const uint32_t lane32 = g + v;
-        const uint32_t voxNum = lane32 + (req & 15u) * 32u;
-        int32_t sel = (v < per && lane32 < 32u) ? select_in(rec->mask, voxNum) : -1;
+        const bool mine = v < per && lane32 < 32u;
+        const int32_t src = mine ? (int32_t)lane32 : 0;
+        int32_t sel = __shfl(selL, src, 64);
+        f3 pp = mk(__shfl(pL.x, src, 64), __shfl(pL.y, src, 64), __shfl(pL.z, src, 64));
+        if(!mine) sel = -1;
         float4 term = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         const uint32_t r0 = c.cRays, m0 = c.cMap, v0 = c.cVox, h0 = c.cHits;
-        DVoxel me = {};
         if(sel >= 0)
         {
-            me = decompress(sv.voxels[rec->voxelIndex + voxNum]);
-            if(!(sv.mats[me.material].specular < 1.0f)) sel = -1;     // no diffuse term
-        }
-        if(sel >= 0)
-        {
-            f3 pp = voxel_origin(rec, sel, me.normal);
             c.lastVoxID = 255u; c.lastVoxRefract = 1.0f;
             DVoxel hv = {};
             f3 ca;
