@@ -323,7 +323,9 @@ void k_direct(DNlightParams P)
     const uint32_t nreq = *P.numRequests;
     const uint32_t per = 64u / N, v = lane / N, k = lane - v * N;
     const f3 dir = ld3(P.directRays[k]), inv = ld3(P.directRays[k] + 3), sun = ld3(sv.sunStrength);
-    const uint32_t* voxRows = maskRows + (threadIdx.x - k) * DN_MASK_ROW;   // row of this voxel's first direction
+    // row of this voxel's first direction (lanes past the last whole voxel get
+    // their wave's first row; they never sum)
+    const uint32_t* voxRows = maskRows + (v < per ? threadIdx.x - k : threadIdx.x - lane) * DN_MASK_ROW;
     for(;;)
     {
         uint32_t r = 0;
@@ -374,7 +376,7 @@ void k_direct_batch(DNbatchParams B, const DNlightParams* __restrict__ scenes)
             const uint32_t N = P.numDirectRays;
             const uint32_t per = 64u / N, v = lane / N, k = lane - v * N;
             const f3 dir = ld3(P.directRays[k]), inv = ld3(P.directRays[k] + 3), sun = ld3(sv.sunStrength);
-            const uint32_t* voxRows = maskRows + (threadIdx.x - k) * DN_MASK_ROW;
+            const uint32_t* voxRows = maskRows + (v < per ? threadIdx.x - k : threadIdx.x - lane) * DN_MASK_ROW;
             const uint32_t nreq = *P.numRequests;
             const uint32_t r0 = (u - pre[lo]) * B.unit;
             for(uint32_t j = 0; j < B.unit && r0 + j < nreq; j++)
