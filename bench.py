@@ -326,7 +326,8 @@ def bench_cache(args) -> int:
     n = vol.fetch_requests().size
     slots = n * 32
     rc = RadianceCache(vol, samplesPerFrame=args.cache_samples, seed=1, lr=args.cache_lr, emaDecay=args.cache_ema,
-                       nMax=args.cache_nmax)
+                       nMax=args.cache_nmax, log2Table=args.cache_table, lrDecaySteps=args.cache_decay_steps,
+                       lrDecayPower=args.cache_decay_power)
     R, F = args.ref_samples, args.frames
     acc = DeviceBuffer(slots * 32)
     acc.zero()
@@ -406,12 +407,17 @@ def bench_cache(args) -> int:
         "ms_per_step": 1e3 * (t_train + t_pred) / F, "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16 (MLP operands), f32 accumulate", "data": "synthetic",
         "config": {"workload": "16x16x16 chunks (128^3 voxels) fBm ~35%, every exposed chunk lit per frame, "
-                               f"16 direct rays, 1 diffuse sample; cache: 16-level hash grid (2^19), MLP 3x64, "
-                               f"Adam lr {args.cache_lr}, {args.cache_steps} step(s) on {args.cache_samples} "
+                               f"16 direct rays, 1 diffuse sample; cache: 16-level hash grid (2^{args.cache_table}), MLP 3x64, "
+                               f"Adam lr {args.cache_lr}"
+                               + (f" / (1 + step / {args.cache_decay_steps:g})^{args.cache_decay_power:g}"
+                                  if args.cache_decay_steps > 0 else "")
+                               + f", {args.cache_steps} step(s) on {args.cache_samples} "
                                f"samples/frame, predictions from the parameter average (decay {args.cache_ema}); "
                                f"reference: {R} independent samples per voxel, FP64",
                    "voxels": int(valid.sum()), "parallelism": "single GPU"},
         "frames_to_1pct": {"cache": fc, "plain": fp},
+        "frames_to": {f"{100 * thr:g}%": {"cache": first_below(curve_c, thr), "plain": first_below(curve_p, thr)}
+                      for thr in (0.10, 0.05, 0.03, 0.02, 0.015, 0.01)},
         "rel_rmse_at": {str(m): {"plain": curve_p[m - 1], "cache": curve_c[m - 1]} for m in marks},
         "after_history_reset": {
             "reset_at_frame": F0, "warm_cache_rel_rmse": warm,
@@ -441,13 +447,17 @@ def main() -> int:
     ap.add_argument("--frames", type=int, default=600, help="frames of the edits and cache workloads")
     ap.add_argument("--edit-api", choices=["batch", "sphere"], default="batch",
                     help="edits workload: one DNX_shape_spheres call per frame (device) or DN_shape_sphere per edit (host)")
-    ap.add_argument("--ref-samples", type=int, default=4096,
+    ap.add_argument("--ref-samples", type=int, default=65536,
                     help="independent samples per voxel of the cache workload's FP64 reference")
-    ap.add_argument("--cache-samples", type=int, default=65536, help="cache training samples per frame")
+    ap.add_argument("--cache-samples", type=int, default=1 << 20, help="cache training samples per frame")
     ap.add_argument("--cache-steps", type=int, default=1, help="cache Adam steps per frame")
     ap.add_argument("--cache-lr", type=float, default=1e-2, help="cache Adam learning rate")
-    ap.add_argument("--cache-ema", type=float, default=0.95, help="decay of the cache's parameter average")
+    ap.add_argument("--cache-ema", type=float, default=0.99, help="decay of the cache's parameter average")
     ap.add_argument("--cache-nmax", type=int, default=256, help="finest hash-grid resolution over the map")
+    ap.add_argument("--cache-table", type=int, default=19, help="log2 of the hash-table entries per level")
+    ap.add_argument("--cache-decay-steps", type=float, default=0.0,
+                    help="cache learning-rate decay: lr / (1 + step / D)^P (0: constant)")
+    ap.add_argument("--cache-decay-power", type=float, default=0.5, help="P of --cache-decay-steps")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",

@@ -127,6 +127,9 @@ bool adam_step(DNXradianceCache* rc)
     rc->adamStep++;
     DNrcAdam o;
     o.lr = rc->cfg.lr;
+    if(rc->cfg.lrDecaySteps > 0.0f)
+        o.lr = (float)((double)rc->cfg.lr / std::pow(1.0 + (double)(rc->adamStep - 1) / (double)rc->cfg.lrDecaySteps,
+                                                     (double)rc->cfg.lrDecayPower));
     o.beta1 = rc->cfg.beta1;
     o.beta2 = rc->cfg.beta2;
     o.eps = rc->cfg.eps;
@@ -163,6 +166,8 @@ extern "C" void DNX_rc_default_config(DNXrcConfig* c)
     c->samplesPerFrame = 65536;
     c->seed = 1;
     c->emaDecay = 0.95f;
+    c->lrDecaySteps = 0.0f;
+    c->lrDecayPower = 0.5f;
 }
 
 extern "C" void DNX_rc_destroy(DNXradianceCache* rc)
@@ -187,11 +192,12 @@ extern "C" DNXradianceCache* DNX_rc_create(DNvolume* vol, const DNXrcConfig* con
     DNX_rc_default_config(&c);
     if(config) c = *config;
     if(c.log2Table < 4 || c.log2Table > 24 || c.nMin < 1 || c.nMax < c.nMin || (c.samplesPerFrame & 15u) ||
-       c.samplesPerFrame == 0 || !(c.lr > 0.0f) || !(c.emaDecay >= 0.0f && c.emaDecay < 1.0f))
+       c.samplesPerFrame == 0 || !(c.lr > 0.0f) || !(c.emaDecay >= 0.0f && c.emaDecay < 1.0f) ||
+       !(c.lrDecaySteps >= 0.0f) || !(c.lrDecayPower >= 0.0f))
     {
         dn_msg(DN_MESSAGE_SHADER, DN_MESSAGE_ERROR,
                "radiance cache config: log2Table in [4, 24], 1 <= nMin <= nMax, samplesPerFrame a positive "
-               "multiple of 16, lr > 0, emaDecay in [0, 1)");
+               "multiple of 16, lr > 0, emaDecay in [0, 1), lrDecaySteps >= 0, lrDecayPower >= 0");
         return nullptr;
     }
     DNXradianceCache* rc = new DNXradianceCache();

@@ -423,6 +423,8 @@ typedef struct DNXrcConfig {
     uint32_t seed;             /* weight initialisation and sample draws */
     float emaDecay;            /* predictions use a debiased moving average of the parameters over the
                                   Adam steps with this decay (0.95); 0: the latest parameters */
+    float lrDecaySteps;        /* step t (from 1) uses lr / (1 + (t - 1) / lrDecaySteps)^lrDecayPower; */
+    float lrDecayPower;        /* lrDecaySteps 0 (default): constant lr. Default power 0.5 */
 } DNXrcConfig;
 void DNX_rc_default_config(DNXrcConfig* config);
 DNXradianceCache* DNX_rc_create(DNvolume* vol, const DNXrcConfig* config /* NULL: defaults */);
