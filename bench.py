@@ -227,7 +227,7 @@ def bench_edits(args) -> int:
     ctrl = [(1.0 + 14.0 * float(uniform01(k, 0, seed=4)), 7.0 + 6.0 * float(uniform01(k, 1, seed=4)),
              1.0 + 14.0 * float(uniform01(k, 2, seed=4))) for k in range(8)]
     n_mats = len(sc.materials) - 1
-    frames = args.frames
+    frames = args.frames or 600
     # the edit stream is input data: drawn for every frame before timing starts
     kk, jj = np.meshgrid(np.arange(256), np.arange(6), indexing="ij")
     edit_u = [uniform01(kk, jj, seed=4 + f).tolist()
@@ -327,8 +327,8 @@ def bench_cache(args) -> int:
     slots = n * 32
     rc = RadianceCache(vol, samplesPerFrame=args.cache_samples, seed=1, lr=args.cache_lr, emaDecay=args.cache_ema,
                        nMax=args.cache_nmax, log2Table=args.cache_table, lrDecaySteps=args.cache_decay_steps,
-                       lrDecayPower=args.cache_decay_power)
-    R, F = args.ref_samples, args.frames
+                       lrDecayPower=args.cache_decay_power, emaGrowth=args.cache_ema_growth)
+    R, F = args.ref_samples, args.frames or 4096
     acc = DeviceBuffer(slots * 32)
     acc.zero()
     t0 = time.perf_counter()
@@ -400,7 +400,7 @@ def bench_cache(args) -> int:
     catch_up = first_below(curve_cold, warm) if warm is not None else None
 
     fp, fc = first_below(curve_p), first_below(curve_c)
-    marks = [m for m in (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048) if m <= F]
+    marks = [m for m in (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096) if m <= F]
     result = {
         "metric": "frames to 1% relative RMSE: radiance cache vs plain accumulation",
         "value": fc, "unit": "frames", "n_gpus": 1, "steps": F, "warmup": 0,
@@ -411,8 +411,10 @@ def bench_cache(args) -> int:
                                f"Adam lr {args.cache_lr}"
                                + (f" / (1 + step / {args.cache_decay_steps:g})^{args.cache_decay_power:g}"
                                   if args.cache_decay_steps > 0 else "")
-                               + f", {args.cache_steps} step(s) on {args.cache_samples} "
-                               f"samples/frame, predictions from the parameter average (decay {args.cache_ema}); "
+                               + f", {args.cache_steps} step(s) per frame, each on {args.cache_samples} fresh "
+                               f"samples, predictions from the parameter average (decay {args.cache_ema}"
+                               + (f", window growing as step / {args.cache_ema_growth:g}" if args.cache_ema_growth > 0
+                                  else "") + "); "
                                f"reference: {R} independent samples per voxel, FP64",
                    "voxels": int(valid.sum()), "parallelism": "single GPU"},
         "frames_to_1pct": {"cache": fc, "plain": fp},
@@ -444,20 +446,24 @@ def main() -> int:
                     help="frame: the full-frame pipeline (default); batch: the batched uneven-volume lighting; "
                          "edits: dynamic edit streaming; cache: radiance cache vs plain accumulation")
     ap.add_argument("--scenes", type=int, default=384, help="volumes of the batch workload")
-    ap.add_argument("--frames", type=int, default=600, help="frames of the edits and cache workloads")
+    ap.add_argument("--frames", type=int, default=None,
+                    help="frames of the edits (default 600) and cache (default 4096) workloads")
     ap.add_argument("--edit-api", choices=["batch", "sphere"], default="batch",
                     help="edits workload: one DNX_shape_spheres call per frame (device) or DN_shape_sphere per edit (host)")
-    ap.add_argument("--ref-samples", type=int, default=65536,
+    ap.add_argument("--ref-samples", type=int, default=262144,
                     help="independent samples per voxel of the cache workload's FP64 reference")
-    ap.add_argument("--cache-samples", type=int, default=1 << 20, help="cache training samples per frame")
-    ap.add_argument("--cache-steps", type=int, default=1, help="cache Adam steps per frame")
+    ap.add_argument("--cache-samples", type=int, default=1 << 17, help="cache training samples per step")
+    ap.add_argument("--cache-steps", type=int, default=8,
+                    help="cache Adam steps per frame, each on --cache-samples fresh samples")
     ap.add_argument("--cache-lr", type=float, default=1e-2, help="cache Adam learning rate")
-    ap.add_argument("--cache-ema", type=float, default=0.99, help="decay of the cache's parameter average")
+    ap.add_argument("--cache-ema", type=float, default=0.9994, help="decay of the cache's parameter average")
     ap.add_argument("--cache-nmax", type=int, default=256, help="finest hash-grid resolution over the map")
     ap.add_argument("--cache-table", type=int, default=19, help="log2 of the hash-table entries per level")
-    ap.add_argument("--cache-decay-steps", type=float, default=0.0,
+    ap.add_argument("--cache-decay-steps", type=float, default=256.0,
                     help="cache learning-rate decay: lr / (1 + step / D)^P (0: constant)")
-    ap.add_argument("--cache-decay-power", type=float, default=0.5, help="P of --cache-decay-steps")
+    ap.add_argument("--cache-decay-power", type=float, default=1.0, help="P of --cache-decay-steps")
+    ap.add_argument("--cache-ema-growth", type=float, default=8.0,
+                    help="cache parameter average: decay min(ema, 1 - G / step) while the window grows (0: off)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",

@@ -120,7 +120,7 @@ class DNXrcConfig(Structure):
     _fields_ = [("log2Table", c_uint32), ("nMin", c_uint32), ("nMax", c_uint32), ("extent", c_float),
                 ("lr", c_float), ("beta1", c_float), ("beta2", c_float), ("eps", c_float),
                 ("samplesPerFrame", c_uint32), ("seed", c_uint32), ("emaDecay", c_float),
-                ("lrDecaySteps", c_float), ("lrDecayPower", c_float)]
+                ("lrDecaySteps", c_float), ("lrDecayPower", c_float), ("emaGrowth", c_float)]
 
 
 MSG_CB = ctypes.CFUNCTYPE(None, c_int, c_int, c_char_p)

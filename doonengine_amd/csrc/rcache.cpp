@@ -4,6 +4,7 @@
 #include "doon/doon.h"
 #include "dn_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -23,6 +24,7 @@ struct DNXradianceCache {
     uint16_t* dQueryX0 = nullptr;
     size_t queryCap = 0;
     uint32_t adamStep = 0;
+    double emaKeep = 1.0;         // product of the average's decays so far (its debiasing)
     uint32_t frame = 0;
 };
 
@@ -92,6 +94,7 @@ bool upload_params(DNXradianceCache* rc, const float* w, const float* tab)
         good = good && ok(hipMemsetAsync(p, 0, sizeof(float) * tf, rc->stream), "clear");
     good = good && ok(hipStreamSynchronize(rc->stream), "upload");
     rc->adamStep = 0;
+    rc->emaKeep = 1.0;
     return good;
 }
 
@@ -135,8 +138,14 @@ bool adam_step(DNXradianceCache* rc)
     o.eps = rc->cfg.eps;
     o.bias1 = (float)(1.0 - std::pow((double)rc->cfg.beta1, (double)rc->adamStep));
     o.bias2 = (float)(1.0 - std::pow((double)rc->cfg.beta2, (double)rc->adamStep));
-    o.emaDecay = rc->cfg.emaDecay;
-    o.emaScale = (float)(1.0 / (1.0 - std::pow((double)rc->cfg.emaDecay, (double)rc->adamStep)));
+    // step t's decay: emaDecay, or while the window grows 1 - emaGrowth / t
+    // (an average over about the last t / emaGrowth steps), at least 0. With
+    // decays d_1..d_t the average's weights sum to 1 - d_1 ... d_t.
+    double d = (double)rc->cfg.emaDecay;
+    if(rc->cfg.emaGrowth > 0.0f) d = std::min(d, std::max(0.0, 1.0 - (double)rc->cfg.emaGrowth / rc->adamStep));
+    rc->emaKeep *= d;
+    o.emaDecay = (float)d;
+    o.emaScale = (float)(1.0 / (1.0 - rc->emaKeep));
     return dn_rc_adam(rc->net, o, rc->stream) == 0;
 }
 
@@ -168,6 +177,7 @@ extern "C" void DNX_rc_default_config(DNXrcConfig* c)
     c->emaDecay = 0.95f;
     c->lrDecaySteps = 0.0f;
     c->lrDecayPower = 0.5f;
+    c->emaGrowth = 8.0f;
 }
 
 extern "C" void DNX_rc_destroy(DNXradianceCache* rc)
@@ -193,11 +203,11 @@ extern "C" DNXradianceCache* DNX_rc_create(DNvolume* vol, const DNXrcConfig* con
     if(config) c = *config;
     if(c.log2Table < 4 || c.log2Table > 24 || c.nMin < 1 || c.nMax < c.nMin || (c.samplesPerFrame & 15u) ||
        c.samplesPerFrame == 0 || !(c.lr > 0.0f) || !(c.emaDecay >= 0.0f && c.emaDecay < 1.0f) ||
-       !(c.lrDecaySteps >= 0.0f) || !(c.lrDecayPower >= 0.0f))
+       !(c.lrDecaySteps >= 0.0f) || !(c.lrDecayPower >= 0.0f) || !(c.emaGrowth >= 0.0f))
     {
         dn_msg(DN_MESSAGE_SHADER, DN_MESSAGE_ERROR,
                "radiance cache config: log2Table in [4, 24], 1 <= nMin <= nMax, samplesPerFrame a positive "
-               "multiple of 16, lr > 0, emaDecay in [0, 1), lrDecaySteps >= 0, lrDecayPower >= 0");
+               "multiple of 16, lr > 0, emaDecay in [0, 1), lrDecaySteps, lrDecayPower, emaGrowth >= 0");
         return nullptr;
     }
     DNXradianceCache* rc = new DNXradianceCache();
@@ -302,12 +312,16 @@ extern "C" bool DNX_rc_train_frame(DNXradianceCache* rc, float time, int steps)
     if(!prepare_lighting(rc->vol, 1, 1000, time, p, maxReq)) return false;
     const uint32_t n = rc->cfg.samplesPerFrame;
     if(!ensure_train(rc, n)) return false;
-    const uint32_t seed = rc->cfg.seed * 0x9E3779B1u + rc->frame * 0x85EBCA77u + 0x632BE5ABu;
-    rc->frame++;
-    if(dn_rc_samples(p, maxReq, n, seed, rc->dSamples, rc->stream) != 0) return false;
-    rc->lastSamples = n;
+    // every step on samples of its own (a step's draws depend on the frame
+    // counter, which advances per step)
     for(int s = 0; s < steps; s++)
+    {
+        const uint32_t seed = rc->cfg.seed * 0x9E3779B1u + rc->frame * 0x85EBCA77u + 0x632BE5ABu;
+        rc->frame++;
+        if(dn_rc_samples(p, maxReq, n, seed, rc->dSamples, rc->stream) != 0) return false;
+        rc->lastSamples = n;
         if(dn_rc_train(rc->net, rc->dSamples, n, rc->acts, rc->stream) != 0 || !adam_step(rc)) return false;
+    }
     return true;
 }
 

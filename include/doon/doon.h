@@ -425,12 +425,14 @@ typedef struct DNXrcConfig {
                                   Adam steps with this decay (0.95); 0: the latest parameters */
     float lrDecaySteps;        /* step t (from 1) uses lr / (1 + (t - 1) / lrDecaySteps)^lrDecayPower; */
     float lrDecayPower;        /* lrDecaySteps 0 (default): constant lr. Default power 0.5 */
+    float emaGrowth;           /* > 0 (default 8): step t's average decay is min(emaDecay, 1 - emaGrowth / t),
+                                  a window of about t / emaGrowth steps while it grows; 0: emaDecay */
 } DNXrcConfig;
 void DNX_rc_default_config(DNXrcConfig* config);
 DNXradianceCache* DNX_rc_create(DNvolume* vol, const DNXrcConfig* config /* NULL: defaults */);
 void DNX_rc_destroy(DNXradianceCache* rc);
-/* After DN_sync_gpu: path-traces samplesPerFrame samples on voxels drawn from
- * the volume's current lighting requests, then takes `steps` Adam steps on them. */
+/* After DN_sync_gpu: takes `steps` Adam steps, each on samplesPerFrame fresh
+ * samples path-traced on voxels drawn from the volume's current lighting requests. */
 bool DNX_rc_train_frame(DNXradianceCache* rc, float time, int steps);
 /* Predicted irradiance for every slot (32 per request) of the current request
  * list: device array of 32 * requestCapacity float4 (rgb, 1 for a stored voxel). */
