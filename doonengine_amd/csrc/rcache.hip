@@ -142,22 +142,71 @@ __global__ __launch_bounds__(256) void k_rc_encode(DNrcNet net, const float* sam
     }
 }
 
-// Hash-table gradient: one thread per (sample, level), the level's two
-// feature gradients spread over the 8 corners by their weights.
+// Levels whose table gradients a wave sums per distinct entry before its
+// atomics. Training samples come in runs of 32 voxels of one lighting request
+// (one chunk), so at the coarse levels a wave's 64 samples share a few cells.
+#ifndef RC_AGG_LEVELS
+#define RC_AGG_LEVELS 8u
+#endif
+
+// gt[2 key] += v0 and gt[2 key + 1] += v1 for every active lane, with one
+// atomic pair per distinct key of the wave: a leader lane's key is matched by
+// every lane holding it, their values are summed across the wave, and the
+// leader adds the sums. Each round retires at least the leader.
+__device__ __forceinline__ void wave_add_pairs(float* gt, uint32_t key, float v0, float v1, bool active)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    uint64_t pending = __ballot(active);
+    while(pending)
+    {
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)pending) - 1u;
+        const uint32_t lk = (uint32_t)__shfl((int)key, (int)leader, 64);
+        const bool mine = active && key == lk;
+        float s0 = mine ? v0 : 0.0f, s1 = mine ? v1 : 0.0f;
+        for(int o = 32; o > 0; o >>= 1)
+        {
+            s0 += __shfl_xor(s0, o, 64);
+            s1 += __shfl_xor(s1, o, 64);
+        }
+        if(lane == leader)
+        {
+            atomicAdd(gt + 2u * lk, s0);
+            atomicAdd(gt + 2u * lk + 1u, s1);
+        }
+        pending &= ~__ballot(mine);
+        active = active && !mine;
+    }
+}
+
+// Hash-table gradient: each wave takes 64 consecutive samples at one level
+// and spreads the level's two feature gradients over the 8 corners by their
+// weights.
 __global__ __launch_bounds__(256) void k_rc_table_grad(DNrcNet net, const float* samples, uint32_t n, const float* denc)
 {
     const uint32_t g = blockIdx.x * 256u + threadIdx.x;
-    const uint32_t s = g >> 4, l = g & 15u;
-    if(s >= n) return;
-    const float d0 = denc[(size_t)s * RC_ENC + 2 * l], d1 = denc[(size_t)s * RC_ENC + 2 * l + 1];
-    if(d0 == 0.0f && d1 == 0.0f) return;
-    Corners c;
-    level_corners(net, l, samples + (size_t)s * RC_SAMPLE, c);
-    float* gt = net.tabGrad + ((size_t)l << net.log2T) * 2u;
-    for(int k = 0; k < 8; k++)
+    const uint32_t lane = threadIdx.x & 63u, w = g >> 6;
+    const uint32_t l = w & 15u, s = (w >> 4) * 64u + lane;
+    float d0 = 0.0f, d1 = 0.0f;
+    if(s < n)
     {
-        atomicAdd(gt + 2u * c.idx[k], c.w[k] * d0);
-        atomicAdd(gt + 2u * c.idx[k] + 1u, c.w[k] * d1);
+        d0 = denc[(size_t)s * RC_ENC + 2 * l];
+        d1 = denc[(size_t)s * RC_ENC + 2 * l + 1];
+    }
+    const bool active = d0 != 0.0f || d1 != 0.0f;
+    float* gt = net.tabGrad + ((size_t)l << net.log2T) * 2u;
+    Corners c;
+    level_corners(net, l, samples + (size_t)(s < n ? s : 0u) * RC_SAMPLE, c);
+    if(l < RC_AGG_LEVELS)
+    {
+        for(int k = 0; k < 8; k++) wave_add_pairs(gt, c.idx[k], c.w[k] * d0, c.w[k] * d1, active);
+    }
+    else if(active)
+    {
+        for(int k = 0; k < 8; k++)
+        {
+            atomicAdd(gt + 2u * c.idx[k], c.w[k] * d0);
+            atomicAdd(gt + 2u * c.idx[k] + 1u, c.w[k] * d1);
+        }
     }
 }
 
@@ -485,8 +534,15 @@ __global__ __launch_bounds__(256) void k_rc_samples(DNlightParams P, uint32_t n,
     uint32_t mapIndex = 0, voxNum = 0;
     int32_t sel = -1;
     bool found = false;
+    // first try: slot g & 31 of a request drawn per run of 32 samples (the
+    // runs keep a wave's samples in two chunks); then random slots. Over the
+    // 32 positions of a run every stored voxel is as likely as any other.
     for(uint32_t t = 0; t < 8u && slots && !found; t++)
-        found = slot_voxel(P, pcg(base + t * 0x68E31DA4u) % slots, rec, mapIndex, voxNum, sel);
+    {
+        const uint32_t slot = t == 0 ? (pcg(seed ^ pcg((g >> 5) ^ 0x5BD1E995u)) % (slots >> 5)) * 32u + (g & 31u)
+                                     : pcg(base + t * 0x68E31DA4u) % slots;
+        found = slot_voxel(P, slot, rec, mapIndex, voxNum, sel);
+    }
     float* out = samples + (size_t)g * RC_SAMPLE;
     if(!found)
     {
@@ -640,8 +696,9 @@ int dn_rc_train(const DNrcNet& net, const float* samples, uint32_t n, const DNrc
     hipLaunchKernelGGL(k_rc_encode, dim3(blocks_for((size_t)n * 16u)), dim3(256), 0, stream, net, samples, n, a.x0);
     hipLaunchKernelGGL(k_rc_mlp<true>, dim3(blocks_for((size_t)tiles * 64u, 1024)), dim3(256), 0, stream, net,
                        (const uint16_t*)a.x0, samples, n, 1.0f / (float)n, (float*)nullptr, a);
-    hipLaunchKernelGGL(k_rc_table_grad, dim3(blocks_for((size_t)n * 16u)), dim3(256), 0, stream, net, samples, n,
-                       (const float*)a.denc);
+    // 16 waves (one per level) per 64 samples
+    hipLaunchKernelGGL(k_rc_table_grad, dim3(blocks_for((size_t)((n + 63u) / 64u) * 1024u)), dim3(256), 0, stream, net,
+                       samples, n, (const float*)a.denc);
     uint32_t wb = tiles < 128u ? tiles : 128u;
     hipLaunchKernelGGL(k_rc_wgrad, dim3(wb), dim3(256), 0, stream, net, tiles, a);
     return (int)hipGetLastError();
