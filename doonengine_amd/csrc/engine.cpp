@@ -165,6 +165,7 @@ struct DNgpuVolume {
     size_t hEditCap = 0;
     uint8_t* dEdit = nullptr;          // device in/out staging
     size_t dEditCap = 0;
+    hipEvent_t editEvents = nullptr;   // an edit batch's events are back on the host
 
     // host mirror of device tile state
     std::vector<uint8_t> state;        // 0 none, 1 unloaded, 2 loaded, 3 to load
@@ -767,6 +768,7 @@ extern "C" void DN_delete_volume(DNvolume* vol)
         dfree(g->dVoxels); dfree(g->dVoxelsOut); dfree(g->dMats); dfree(g->dNumRequests); dfree(g->dWorkCounter);
         dfree(g->dCounters); dfree(g->dPoints); dfree(g->dStage); dfree(g->dEdit); dfree(g->dCube); dfree(g->dDirect);
         if(g->hEdit) hipHostFree(g->hEdit);
+        if(g->editEvents) hipEventDestroy(g->editEvents);
         for(int s = 0; s < 2; s++)
         {
             if(g->hStage[s]) hipHostFree(g->hStage[s]);
@@ -1923,7 +1925,9 @@ extern "C" bool DNX_shape_spheres(DNvolume* vol, const DNXsphereEdit* edits, uin
     // (tile, sphere) pairs in the order DN_shape_sphere visits them
     struct Pair { uint32_t tile, sphere; };
     std::vector<Pair> pairs;
+    pairs.reserve((size_t)n * 64u);
     std::vector<DNeditSphere> sph(n);
+    std::vector<float> off[3];          // per axis: each tile's nearest-voxel offset
     for(uint32_t e = 0; e < n; e++)
     {
         const DNXsphereEdit& in = edits[e];
@@ -1936,12 +1940,18 @@ extern "C" bool DNX_shape_spheres(DNvolume* vol, const DNXsphereEdit* edits, uin
         const DNcompressedVoxel cv = DN_compress_voxel(in.voxel);
         d.carveNormal = cv.normal;
         d.albedo = cv.albedo;
+        const int lo[3] = {b.lo.x, b.lo.y, b.lo.z}, hi[3] = {b.hi.x, b.hi.y, b.hi.z};
+        const float cc[3] = {in.center.x, in.center.y, in.center.z};
+        for(int a = 0; a < 3; a++)
+        {
+            off[a].clear();
+            for(int m = lo[a]; m <= hi[a]; m++) off[a].push_back(nearest_offset(m * 8, cc[a]));
+        }
         for(int mZ = b.lo.z; mZ <= b.hi.z; mZ++)
         for(int mY = b.lo.y; mY <= b.hi.y; mY++)
         for(int mX = b.lo.x; mX <= b.hi.x; mX++)
         {
-            const float nx = nearest_offset(mX * 8, in.center.x), ny = nearest_offset(mY * 8, in.center.y),
-                        nz = nearest_offset(mZ * 8, in.center.z);
+            const float nx = off[0][mX - lo[0]], ny = off[1][mY - lo[1]], nz = off[2][mZ - lo[2]];
             if(!sphere_touches(b, nx * nx + ny * ny + nz * nz)) continue;
             pairs.push_back({flat(vol, DNivec3{mX, mY, mZ}), e});
         }
@@ -1978,9 +1988,11 @@ extern "C" bool DNX_shape_spheres(DNvolume* vol, const DNXsphereEdit* edits, uin
                  oStart = align16(oInfo + 4u * U), oPSph = align16(oStart + 4u * (U + 1)),
                  oPSeq = align16(oPSph + 4u * P), oStale = align16(oPSeq + 4u * P),
                  oStaleW = align16(oStale + 4u * S), inBytes = align16(oStaleW + 4096u * S);
-    const size_t oEv = inBytes, oOut = align16(oEv + P), oGIds = align16(oOut + 4u * U),
-                 oGW = align16(oGIds + 4u * U), allBytes = align16(oGW + 4096u * U);
+    const size_t oEv = inBytes, oOut = align16(oEv + P), oGW = align16(oOut + 4u * U),
+                 allBytes = align16(oGW + 4096u * U);
     if(!ensure_edit_stage(g, allBytes, allBytes)) return false;
+    if(!g->editEvents && !HIP_OK(hipEventCreateWithFlags(&g->editEvents, hipEventDisableTiming), "edit event"))
+        return false;
     uint8_t* h = g->hEdit;
     uint8_t* d = g->dEdit;
     memcpy(h + oSph, sph.data(), sizeof(DNeditSphere) * n);
@@ -2024,9 +2036,18 @@ extern "C" bool DNX_shape_spheres(DNvolume* vol, const DNXsphereEdit* edits, uin
     ok = ok && dn_launch_edit_scatter(g->dEditWords, (const uint32_t*)(d + oStale), (const uint2*)(d + oStaleW), S,
                                       g->stream) == 0;
     ok = ok && dn_launch_sphere_edits(ep, U, g->stream) == 0;
-    ok = ok && HIP_OK(hipMemcpyAsync(h + oEv, d + oEv, oGIds - oEv, hipMemcpyDeviceToHost, g->stream), "edit readback");
-    ok = ok && HIP_OK(hipStreamSynchronize(g->stream), "sphere edits");
-    if(!ok) return false;
+    // the events and tile states come back first; the final words of every
+    // touched tile follow while the host replays the events
+    ok = ok && HIP_OK(hipMemcpyAsync(h + oEv, d + oEv, oGW - oEv, hipMemcpyDeviceToHost, g->stream), "edit readback");
+    ok = ok && HIP_OK(hipEventRecord(g->editEvents, g->stream), "edit event");
+    ok = ok && dn_launch_edit_gather(g->dEditWords, ep.tileIds, (uint2*)(d + oGW), U, g->stream) == 0;
+    ok = ok && HIP_OK(hipMemcpyAsync(h + oGW, d + oGW, 4096u * U, hipMemcpyDeviceToHost, g->stream), "edit readback");
+    ok = ok && HIP_OK(hipEventSynchronize(g->editEvents), "sphere edits");
+    if(!ok)
+    {
+        hipStreamSynchronize(g->stream);
+        return false;
+    }
 
     // replay the chunk events in the reference's order
     const uint8_t* ev = h + oEv;
@@ -2034,6 +2055,11 @@ extern "C" bool DNX_shape_spheres(DNvolume* vol, const DNXsphereEdit* edits, uin
     {
         if(!ev[q]) continue;
         const uint32_t t = pairs[q].tile;
+        if(!(ev[q] & (DN_EDIT_ADDED | DN_EDIT_REMOVED)))
+        {
+            mark_dirty(g, t);
+            continue;
+        }
         const DNivec3 mp = {(int)(t % vol->mapSize.x), (int)((t / vol->mapSize.x) % vol->mapSize.y),
                             (int)(t / (vol->mapSize.x * vol->mapSize.y))};
         if(ev[q] & DN_EDIT_ADDED)
@@ -2041,17 +2067,15 @@ extern "C" bool DNX_shape_spheres(DNvolume* vol, const DNXsphereEdit* edits, uin
             if(DN_add_chunk(vol, mp) < 0)
             {
                 dn_msg(DN_MESSAGE_CPU_MEMORY, DN_MESSAGE_ERROR, "DNX_shape_spheres: no chunk memory for an edit");
+                hipStreamSynchronize(g->stream);
                 return false;
             }
         }
-        else if(ev[q] & DN_EDIT_REMOVED)
-            DN_remove_chunk(vol, mp);
         else
-            mark_dirty(g, t);
+            DN_remove_chunk(vol, mp);
     }
     // final words of the tiles that hold a chunk and were written
     const uint32_t* out = (const uint32_t*)(h + oOut);
-    uint32_t* gIds = (uint32_t*)(h + oGIds);
     std::vector<uint32_t> gSlot;
     for(uint32_t u = 0; u < U; u++)
     {
@@ -2060,32 +2084,36 @@ extern "C" bool DNX_shape_spheres(DNvolume* vol, const DNXsphereEdit* edits, uin
         {
             dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR, "DNX_shape_spheres: chunk state mismatch at tile %u",
                    tileIds[u]);
+            hipStreamSynchronize(g->stream);
             return false;
         }
-        if(present && modified)
-        {
-            gIds[gSlot.size()] = tileIds[u];
-            gSlot.push_back(u);
-        }
+        if(present && modified) gSlot.push_back(u);
     }
+    if(!HIP_OK(hipStreamSynchronize(g->stream), "sphere edits")) return false;
+    // copy the written chunks' words into the CPU chunks, split over a few
+    // threads when there are many (each chunk is 4 KiB)
     const uint32_t G = (uint32_t)gSlot.size();
-    if(G)
-    {
-        ok = HIP_OK(hipMemcpyAsync(d + oGIds, h + oGIds, 4u * G, hipMemcpyHostToDevice, g->stream), "edit gather");
-        ok = ok && dn_launch_edit_gather(g->dEditWords, (const uint32_t*)(d + oGIds), (uint2*)(d + oGW), G,
-                                         g->stream) == 0;
-        ok = ok && HIP_OK(hipMemcpyAsync(h + oGW, d + oGW, 4096u * G, hipMemcpyDeviceToHost, g->stream),
-                          "edit readback");
-        ok = ok && HIP_OK(hipStreamSynchronize(g->stream), "sphere edits");
-        if(!ok) return false;
-        for(uint32_t i = 0; i < G; i++)
+    auto copy_range = [&](uint32_t a, uint32_t b) {
+        for(uint32_t i = a; i < b; i++)
         {
             const uint32_t u = gSlot[i];
             DNchunk& ch = vol->chunks[vol->map[tileIds[u]].chunkIndex];
-            memcpy(ch.voxels, h + oGW + 4096u * i, 4096u);
+            memcpy(ch.voxels, h + oGW + 4096u * u, 4096u);
             ch.numVoxels = out[u] & 0x3FFFFFFFu;
             ch.updated = true;
         }
+    };
+    const uint32_t workers = G >= 2048u ? 8u : (G >= 512u ? 4u : 1u);
+    if(workers == 1u)
+        copy_range(0, G);
+    else
+    {
+        std::vector<std::thread> pool;
+        const uint32_t per = (G + workers - 1u) / workers;
+        for(uint32_t w = 1; w < workers; w++)
+            pool.emplace_back(copy_range, std::min(G, w * per), std::min(G, (w + 1u) * per));
+        copy_range(0, std::min(G, per));
+        for(std::thread& t : pool) t.join();
     }
     for(uint32_t u = 0; u < U; u++) g->editValid[tileIds[u]] = 1;
     return true;
