@@ -4,6 +4,10 @@
   table agree with the oracle's bf16-operand restatement (which the CPU suite
   pins against finite differences) to within accumulation-order differences.
 - The first Adam step moves every parameter by lr * g / (|g| + eps).
+- The learning-rate decay scales step t by 1 / (1 + (t - 1) / D)^P, and the
+  growing parameter average (decay min(emaDecay, 1 - G / t)) predicts with
+  exactly the parameters it should: the latest ones while G / t >= 1, then the
+  FP32 average of the steps so far.
 - On a small scene, after the same number of frames, the trained cache's
   per-voxel irradiance is closer to a 1024-frame FP64-accumulated reference
   than the plain running mean is.
@@ -119,6 +123,48 @@ def test_adam_first_step(small_volume):
             assert np.all(p1[~(np.abs(gg) > 0)] == p0[~(np.abs(gg) > 0)])
     finally:
         rc.close()
+
+
+def test_lr_decay_and_growing_average(small_volume):
+    vol, _ = small_volume
+    cfg = orc.Config(log2_table=LOG2T, extent=128.0)
+    rng = np.random.default_rng(23)
+    mats, table = orc.init_params(cfg, seed=4)
+    B = 256
+    smp = make_samples(rng, B)
+    lr, D, G, ema = 0.01, 4.0, 8.0, 0.95
+    # beta1 = beta2 = 0: step t moves a parameter by lr_t * g / (|g| + eps), so
+    # the typical move of the weights is lr_t itself
+    rc = RadianceCache(vol, log2Table=LOG2T, extent=128.0, lr=lr, beta1=0.0, beta2=0.0, lrDecaySteps=D,
+                       lrDecayPower=1.0, emaGrowth=G, emaDecay=ema)
+    ref = RadianceCache(vol, log2Table=LOG2T, extent=128.0)
+    try:
+        buf = DeviceBuffer.from_array(smp)
+        rc.set_params(rc_pack_weights(mats), table.reshape(-1))
+        params = [rc.get_params()[:2]]
+        for _ in range(9):
+            rc.train_samples(buf, B, step=True)
+            params.append(rc.get_params()[:2])
+        for t in (1, 2, 5, 9):
+            d = np.abs(params[t][0] - params[t - 1][0])
+            moved = d > 0
+            assert moved.sum() > 1000
+            assert abs(float(np.median(d[moved])) / (lr / (1.0 + (t - 1) / D)) - 1.0) < 1e-3, t
+        # the average as the Adam kernel keeps it, in FP32
+        ew, et = params[1]
+        for t in range(2, 10):
+            dk = np.float32(min(float(np.float32(ema)), max(0.0, 1.0 - G / t)))
+            ew = dk * ew + (np.float32(1.0) - dk) * params[t][0]
+            et = dk * et + (np.float32(1.0) - dk) * params[t][1]
+        assert not np.array_equal(ew, params[9][0])         # step 9 is the first real average
+        out_rc, out_ref = DeviceBuffer(B * 16), DeviceBuffer(B * 16)
+        rc.predict_samples(buf, B, out_rc)
+        ref.set_params(ew, et)
+        ref.predict_samples(buf, B, out_ref)
+        assert np.array_equal(out_rc.download(np.float32, (B, 4)), out_ref.download(np.float32, (B, 4)))
+    finally:
+        rc.close()
+        ref.close()
 
 
 def test_warm_cache_beats_cold_accumulation(gpu):
