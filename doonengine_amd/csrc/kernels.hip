@@ -274,35 +274,37 @@ __device__ __forceinline__ void direct_request(const DNlightParams& P, const DNs
             if(c.lastVoxID != 255u) term.w += 2.0f;    // ended inside a transparent voxel
         }
         __builtin_amdgcn_wave_barrier();      // every lane's walk is done with its row
-        row[0] = __float_as_uint(term.x); row[1] = __float_as_uint(term.y);
-        row[2] = __float_as_uint(term.z); row[3] = __float_as_uint(term.w);
+        // the row: the term's x, y, z (0 when the ray does not reach the sun),
+        // whether the ray ended inside a transparent voxel, the ray's counters
+        row[0] = __float_as_uint(term.x); row[1] = __float_as_uint(term.y); row[2] = __float_as_uint(term.z);
+        row[3] = term.w >= 2.0f ? 1u : 0u;
         row[4] = c.cRays - r0; row[5] = c.cMap - m0; row[6] = c.cVox - v0; row[7] = c.cHits - h0;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if(sel >= 0 && k == 0)
-        {
-            f3 acc = splat(0.0f);
-            bool stale = false;
-            uint4 n = make_uint4(0u, 0u, 0u, 0u);
-            for(uint32_t j = 0; j < N; j++)
+        // Lane k of a voxel sums word k of the voxel's N rows (a lane takes
+        // words k, k + N, ... when N < 8), which is word k of its DNdirectTerm:
+        // x, y, z in direction order as shadow_ray adds them (a term that
+        // does not reach the sun is +0 and leaves the sum as it is; every sum
+        // is >= +0), divided by N; the transparent-end flags (any); the
+        // counters (integers, any order).
+        if(sel >= 0)
+            for(uint32_t wd = k; wd < 8u; wd += N)
             {
-                const uint32_t* q0 = voxRows + j * DN_MASK_ROW;
-                const float4 q = make_float4(__uint_as_float(q0[0]), __uint_as_float(q0[1]), __uint_as_float(q0[2]),
-                                             __uint_as_float(q0[3]));
-                const uint4 qc = make_uint4(q0[4], q0[5], q0[6], q0[7]);
-                if(q.w == 1.0f || q.w == 3.0f) acc = acc + mk(q.x, q.y, q.z);
-                stale = stale || q.w >= 2.0f;
-                n.x += qc.x; n.y += qc.y; n.z += qc.z; n.w += qc.w;
+                float fs = 0.0f;
+                uint32_t us = 0u;
+                const uint32_t* q = voxRows + wd;
+                for(uint32_t j = 0; j < N; j++, q += DN_MASK_ROW)
+                {
+                    const uint32_t x = *q;
+                    fs = fs + __uint_as_float(x);
+                    us += x;
+                }
+                const size_t slot = (size_t)r * 32u + lane32;
+                if(slot < P.directSlots)
+                    reinterpret_cast<uint32_t*>(P.directOut + slot)[wd] =
+                        wd < 3u ? __float_as_uint(fs / (float)N) : (wd == 3u ? __float_as_uint(us ? 1.0f : 0.0f) : us);
             }
-            acc = acc / (float)N;
-            const size_t slot = (size_t)r * 32u + lane32;
-            if(slot < P.directSlots)
-            {
-                P.directOut[slot].term = make_float4(acc.x, acc.y, acc.z, stale ? 1.0f : 0.0f);
-                P.directOut[slot].counts = n;
-            }
-        }
         __builtin_amdgcn_wave_barrier();
     }
 }
