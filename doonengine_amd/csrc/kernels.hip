@@ -224,7 +224,8 @@ void k_lighting(DNlightParams P)
 // >= 1) are skipped. A wave takes one request (32 slots) at a time.
 // Waves per SIMD the direct-ray kernel is compiled for: 80 VGPRs, a few
 // spills outside the walks. Measured on the full-frame workload: 4 waves
-// 6.55 ms, 5 waves 5.88, 6 waves 5.49, 7 waves (spilling in the walks) 5.93.
+// 6.55 ms, 5 waves 5.88, 6 waves 5.49, 7 waves (spilling in the walks) 5.93;
+// with the sums spread over 8 lanes per voxel, 6 waves 4.65 ms, 7 waves 4.67.
 // The chunk-grid occupancy bitmap is read from L2 here: staging it in LDS
 // (16 KiB per block on that map) would hold the kernel to 4 waves.
 #ifndef DN_DIRECT_WAVES
