@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <set>
 #include <thread>
 #include <vector>
@@ -2108,11 +2109,24 @@ extern "C" bool DNX_shape_spheres(DNvolume* vol, const DNXsphereEdit* edits, uin
         copy_range(0, G);
     else
     {
+        // a thread that cannot be started leaves its range to this one
         std::vector<std::thread> pool;
         const uint32_t per = (G + workers - 1u) / workers;
+        uint32_t done = per;
         for(uint32_t w = 1; w < workers; w++)
-            pool.emplace_back(copy_range, std::min(G, w * per), std::min(G, (w + 1u) * per));
+        {
+            try
+            {
+                pool.emplace_back(copy_range, std::min(G, w * per), std::min(G, (w + 1u) * per));
+            }
+            catch(const std::exception&)
+            {
+                break;
+            }
+            done = std::min(G, (w + 1u) * per);
+        }
         copy_range(0, std::min(G, per));
+        copy_range(done, G);
         for(std::thread& t : pool) t.join();
     }
     for(uint32_t u = 0; u < U; u++) g->editValid[tileIds[u]] = 1;
