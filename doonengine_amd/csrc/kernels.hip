@@ -222,14 +222,20 @@ void k_lighting(DNlightParams P)
 // with the term (the pass adds them whenever it uses it), so the pass's counts
 // are those of tracing in sequence. Voxels without a diffuse term (specular
 // >= 1) are skipped. A wave takes one request (32 slots) at a time.
-// Waves per SIMD the direct-ray kernel is compiled for: 80 VGPRs, a few
-// spills outside the walks. Measured on the full-frame workload: 4 waves
-// 6.55 ms, 5 waves 5.88, 6 waves 5.49, 7 waves (spilling in the walks) 5.93;
-// with the sums spread over 8 lanes per voxel, 6 waves 4.65 ms, 7 waves 4.67.
+// Waves per SIMD the direct-ray kernel is compiled for: 96 VGPRs, no spills.
+// Measured on the full-frame workload (lighting pass) with the sums spread
+// over 8 lanes per voxel: 4 waves 4.73 ms, 5 waves 4.38, 6 waves (80 VGPRs,
+// a few spills) 4.67, 7 waves 4.67. With one summing lane per voxel 6 waves
+// had been best (4 waves 6.55 ms, 5 waves 5.88, 6 waves 5.49, 7 waves 5.93).
 // The chunk-grid occupancy bitmap is read from L2 here: staging it in LDS
 // (16 KiB per block on that map) would hold the kernel to 4 waves.
 #ifndef DN_DIRECT_WAVES
-#define DN_DIRECT_WAVES 6
+#define DN_DIRECT_WAVES 5
+#endif
+// The batched direct pass (384 uneven volumes, 16 directions: 4 voxels per
+// wave) measured 49.0 ms of graph time at 6 waves against 52.0 at 5.
+#ifndef DN_DIRECT_BATCH_WAVES
+#define DN_DIRECT_BATCH_WAVES 6
 #endif
 // The direct rays of one request's 32 slots: lane = voxel * N + direction,
 // 64 / N voxels at a time. `row` is the lane's mask row, `voxRows` the row of
@@ -342,7 +348,7 @@ void k_direct(DNlightParams P)
 // k_direct over the volumes of a batch, with the work units, queues and
 // per-queue counters of k_lighting_batch (its own counters). Volumes whose pass
 // traces its direct rays itself (no directOut) are skipped.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DN_DIRECT_WAVES, DN_DIRECT_WAVES)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DN_DIRECT_BATCH_WAVES, DN_DIRECT_BATCH_WAVES)))
 void k_direct_batch(DNbatchParams B, const DNlightParams* __restrict__ scenes)
 {
     const uint32_t lane = threadIdx.x & 63u;
