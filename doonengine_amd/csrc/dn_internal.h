@@ -271,6 +271,10 @@ int dn_launch_edit_gather(const uint2* words, const uint32_t* tiles, uint2* out,
 struct DNvolume;
 bool prepare_lighting(DNvolume* vol, int numDiffuseSamples, int maxDiffuseSamples, float time, DNlightParams& p,
                       uint32_t& maxRequests);
+// Traces the direct rays of the volume's current requests ahead (k_direct, on
+// the volume's stream) into its direct-term buffer and points p.directIn at
+// it; false (p unchanged) when there are none to trace or on failure.
+bool trace_direct_terms(DNvolume* vol, DNlightParams& p, uint32_t maxRequests);
 
 // Kernel launchers (kernels.hip). All asynchronous on `stream`; they return a
 // hipError_t value.

@@ -1290,6 +1290,22 @@ extern "C" void DN_update_lighting(DNvolume* vol, int numDiffuseSamples, int max
     if(e || e2) dn_msg(DN_MESSAGE_SHADER, DN_MESSAGE_ERROR, "lighting launch failed (%d, %d)", e, e2);
 }
 
+bool trace_direct_terms(DNvolume* vol, DNlightParams& p, uint32_t maxRequests)
+{
+    DNgpuVolume* g = vol->gpu;
+    if(p.numDirectRays == 0 || maxRequests == 0 || !ensure_direct(g, maxRequests)) return false;
+    p.directOut = g->dDirect;
+    p.directSlots = (uint32_t)std::min<size_t>(g->directCap, 0xFFFFFFFFu);
+    const int e = dn_launch_direct(p, maxRequests, g->stream);
+    p.directOut = nullptr;
+    // k_direct's request counter, which the lighting pass's epilogue clears
+    // otherwise (this runs outside a lighting pass and outside graph capture)
+    const bool ok = e == 0 && HIP_OK(hipMemsetAsync(g->dWorkCounter + 4, 0, sizeof(uint32_t), g->stream), "counter");
+    if(ok) p.directIn = g->dDirect;
+    else p.directSlots = 0;
+    return ok;
+}
+
 // ===========================================================================
 // MAP SETTINGS (voxel.c:957-1096)
 

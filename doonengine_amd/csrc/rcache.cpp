@@ -331,6 +331,9 @@ extern "C" bool DNX_rc_predict_requests(DNXradianceCache* rc, float* out, size_t
     DNlightParams p;
     uint32_t maxReq = 0;
     if(!prepare_lighting(rc->vol, 1, 1000, rc->vol->lastTime, p, maxReq)) return false;
+    // the query rows' exact direct terms, traced ahead in direction-major waves
+    // (bit-identical to tracing them per voxel; without them the gather does)
+    trace_direct_terms(rc->vol, p, maxReq);
     size_t slots = (size_t)requestCapacity * 32u;
     if(slots == 0) return true;
     if(slots > 0xFFFFFFF0u || !ensure_query(rc, slots)) return false;

@@ -597,6 +597,7 @@ __global__ __launch_bounds__(256) void k_rc_gather(DNlightParams P, uint32_t slo
         RayCtx c;
         LCtx L;
         site_ctx(P, s, c, L, maskRows);
+        L.direct = (P.directIn && g < P.directSlots) ? P.directIn + g : nullptr;
         exact = ld3(P.ambient);
         shadow_ray(L, v.p, 0, exact);
     }

@@ -167,6 +167,47 @@ def test_lr_decay_and_growing_average(small_volume):
         ref.close()
 
 
+def test_prediction_exact_term_matches_lighting_pass(gpu):
+    # With no diffuse bounces the pass's first-frame estimate is exactly its
+    # ambient + direct term, and an untrained cache (zero output layer)
+    # predicts exactly its query rows' ambient + direct term: the two must
+    # agree bit for bit on every voxel.
+    sc = scenes.config_small(seed=5, chunks=(6, 4, 6))
+    vol = Volume(sc.map_size, min_chunks=256)
+    try:
+        sc.apply(vol)
+        vol.v.diffuseBounceLimit = 0
+        vol.v.numDirectRays = 16
+        vol.request_all()
+        vol.sync()
+        vol.mark_visible(1 << 20)
+        vol.sync()
+        vol.synchronize()
+        n = vol.fetch_requests().size
+        slots = n * 32
+        dbg, idx = DeviceBuffer(slots * 24), DeviceBuffer(slots * 4)
+        vol.lib.DNX_set_lighting_debug_output(vol.ptr, dbg.ptr, idx.ptr, n)
+        vol.update_lighting(1, 1000, 123.456)
+        vol.synchronize()
+        diff = dbg.download(np.float32, (slots, 6))[:, :3]
+        vi = idx.download(np.uint32, (slots,))
+        vol.lib.DNX_set_lighting_debug_output(vol.ptr, None, None, 0)
+        rc = RadianceCache(vol)
+        try:
+            pred = DeviceBuffer(slots * 16)
+            rc.predict_requests(pred, n)
+            vol.synchronize()
+            p = pred.download(np.float32, (slots, 4))
+        finally:
+            rc.close()
+        valid = vi != 0xFFFFFFFF
+        assert valid.sum() > 1000
+        assert np.any(diff[valid] > 0)
+        assert np.array_equal(p[valid, :3], diff[valid])
+    finally:
+        vol.close()
+
+
 def test_warm_cache_beats_cold_accumulation(gpu):
     """What the trained network gains: after a lighting-history reset (an edit,
     or a region seen for the first time) the engine's running mean restarts
