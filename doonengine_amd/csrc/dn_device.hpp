@@ -230,6 +230,8 @@ struct RayCtx {
     uint32_t sMap, sVox;          // trips of the last step_map
     uint32_t cRays, cMap, cVox, cHits;
     uint32_t* maskRow;            // this lane's LDS row for the chunk mask (LDS-mask walks)
+    const uint32_t* sharedRow;    // an LDS row already holding the mask of tile sharedMap
+    uint32_t sharedMap;           // (0xFFFFFFFF: none)
 };
 
 // LDS rows for the chunk masks of the lanes of a block: 17 words per lane (16
@@ -241,6 +243,8 @@ struct RayCtx {
 __device__ __forceinline__ void ctx_init(RayCtx& c)
 {
     c.maskRow = nullptr;
+    c.sharedRow = nullptr;
+    c.sharedMap = 0xFFFFFFFFu;
     c.lastVoxID = 255u; c.lastVoxRefract = 1.0f; c.hitIndex = 0xFFFFFFFFu;
     c.sMap = c.sVox = 0; c.cRays = c.cMap = c.cVox = c.cHits = 0;
 }
@@ -302,13 +306,19 @@ __device__ bool step_chunk(const DNsceneView& s, RayCtx& c, uint32_t mapIndex, f
     const DNchunkRec* rec = s.chunks + mapIndex;
     const uint4* rq = reinterpret_cast<const uint4*>(rec);
     Mask512 m;
+    const uint32_t* mrow = c.maskRow;
     if(LDSM)
     {
-        uint32_t* row = c.maskRow;
-        for(int k = 0; k < 4; k++)
+        if(mapIndex == c.sharedMap)
+            mrow = c.sharedRow;            // staged once for the whole wave
+        else
         {
-            uint4 v = rq[k];
-            row[4 * k + 0] = v.x; row[4 * k + 1] = v.y; row[4 * k + 2] = v.z; row[4 * k + 3] = v.w;
+            uint32_t* row = c.maskRow;
+            for(int k = 0; k < 4; k++)
+            {
+                uint4 v = rq[k];
+                row[4 * k + 0] = v.x; row[4 * k + 1] = v.y; row[4 * k + 2] = v.z; row[4 * k + 3] = v.w;
+            }
         }
     }
     else
@@ -323,12 +333,12 @@ __device__ bool step_chunk(const DNsceneView& s, RayCtx& c, uint32_t mapIndex, f
     {
         c.sVox++;
         uint32_t li = (uint32_t)(d.px + 8 * (d.py + 8 * d.pz));
-        const uint32_t bit = LDSM ? (c.maskRow[li >> 5] >> (li & 31u)) & 1u : mask_bit(m, li);
+        const uint32_t bit = LDSM ? (mrow[li >> 5] >> (li & 31u)) & 1u : mask_bit(m, li);
         if(bit && !ignoreFirst)
         {
             uint4 tail = rq[4];   // partial[0..2], voxelIndex
             uint32_t partial[3] = {tail.x, tail.y, tail.z};
-            uint32_t idx = tail.w + (LDSM ? rank_row(c.maskRow, partial, li) : rank_in(m, partial, li));
+            uint32_t idx = tail.w + (LDSM ? rank_row(mrow, partial, li) : rank_in(m, partial, li));
             uint4 cw = s.voxels[idx];
             vox = decompress(cw);
             const DNmat& mat = s.mats[vox.material];

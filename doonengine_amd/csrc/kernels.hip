@@ -241,11 +241,22 @@ void k_lighting(DNlightParams P)
 // 64 / N voxels at a time. `row` is the lane's mask row, `voxRows` the row of
 // its voxel's first direction.
 __device__ __forceinline__ void direct_request(const DNlightParams& P, const DNsceneView& sv, uint32_t r, RayCtx& c,
-                                               uint32_t* row, const uint32_t* voxRows, uint32_t N, uint32_t per,
-                                               uint32_t v, uint32_t k, f3 dir, f3 inv, f3 sun)
+                                               uint32_t* row, const uint32_t* voxRows, uint32_t* originRow,
+                                               uint32_t N, uint32_t per, uint32_t v, uint32_t k, f3 dir, f3 inv,
+                                               f3 sun)
 {
     const uint32_t req = P.requests[r];
     const DNchunkRec* rec = sv.chunks + (req >> 4);
+    // every ray of the request starts in the request's chunk: its mask is
+    // staged once, in the wave's origin row, for all of them
+    const uint32_t lane = threadIdx.x & 63u;
+    __builtin_amdgcn_wave_barrier();
+    if(lane < 16u) originRow[lane] = rec->mask[lane];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    c.sharedRow = originRow;
+    c.sharedMap = req >> 4;
     // each slot's start point once per request: lane l prepares slot l & 31,
     // the tracing lanes fetch theirs by shuffle
     const uint32_t myNum = (threadIdx.x & 31u) + (req & 15u) * 32u;
@@ -314,6 +325,7 @@ __device__ __forceinline__ void direct_request(const DNlightParams& P, const DNs
             }
         __builtin_amdgcn_wave_barrier();
     }
+    c.sharedMap = 0xFFFFFFFFu;
 }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DN_DIRECT_WAVES, DN_DIRECT_WAVES)))
@@ -323,6 +335,7 @@ void k_direct(DNlightParams P)
     // per lane: the chunk-mask row of its walks, then (between walks) its
     // ray's term and counters for the summing lane of its voxel
     __shared__ uint32_t maskRows[DN_MASK_LDS_WORDS(256)];
+    __shared__ uint32_t originRows[4 * DN_MASK_ROW];     // per wave: the request chunk's mask
     const DNsceneView sv = P.scene;
     const uint32_t N = P.numDirectRays;
     RayCtx c;
@@ -341,7 +354,8 @@ void k_direct(DNlightParams P)
         if(lane == 0) r = atomicAdd(P.workCounter + 4, 1u);
         r = __shfl(r, 0, 64);
         if(r >= nreq) break;
-        direct_request(P, sv, r, c, row, voxRows, N, per, v, k, dir, inv, sun);
+        direct_request(P, sv, r, c, row, voxRows, originRows + (threadIdx.x >> 6) * DN_MASK_ROW, N, per, v, k, dir,
+                       inv, sun);
     }
 }
 
@@ -354,6 +368,7 @@ void k_direct_batch(DNbatchParams B, const DNlightParams* __restrict__ scenes)
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t home = blockIdx.x % B.queues;
     __shared__ uint32_t maskRows[DN_MASK_LDS_WORDS(256)];
+    __shared__ uint32_t originRows[4 * DN_MASK_ROW];     // per wave: the request chunk's mask
     RayCtx c;
     ctx_init(c);
     uint32_t* row = maskRows + threadIdx.x * DN_MASK_ROW;
@@ -389,7 +404,8 @@ void k_direct_batch(DNbatchParams B, const DNlightParams* __restrict__ scenes)
             const uint32_t nreq = *P.numRequests;
             const uint32_t r0 = (u - pre[lo]) * B.unit;
             for(uint32_t j = 0; j < B.unit && r0 + j < nreq; j++)
-                direct_request(P, sv, r0 + j, c, row, voxRows, N, per, v, k, dir, inv, sun);
+                direct_request(P, sv, r0 + j, c, row, voxRows, originRows + (threadIdx.x >> 6) * DN_MASK_ROW, N,
+                               per, v, k, dir, inv, sun);
         }
     }
 }

@@ -55,6 +55,7 @@ def test_deterministic_terms_within_1e4(gpu):
     req = vol.fetch_requests()
     R = len(req)
     dbg, dbi = DeviceBuffer(R * 32 * 24), DeviceBuffer(R * 32 * 4)
+    dbg.zero()                       # slots without a voxel are never written
     vol.lib.DNX_set_lighting_debug_output(vol.ptr, dbg.ptr, dbi.ptr, R)
     s, _ = oracle_scene(o64, vol, sc)
     vol.update_lighting(1, 1000, 300.0)
