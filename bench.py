@@ -221,28 +221,15 @@ def bench_edits(args) -> int:
     vol.request_all()
     vol.sync()
     vol.synchronize()
-    X, Y, Z = (8 * c for c in sc.map_size)
     W, H = sc.width, sc.height
     out = DeviceBuffer(W * H * 16)
     ctrl = [(1.0 + 14.0 * float(uniform01(k, 0, seed=4)), 7.0 + 6.0 * float(uniform01(k, 1, seed=4)),
              1.0 + 14.0 * float(uniform01(k, 2, seed=4))) for k in range(8)]
-    n_mats = len(sc.materials) - 1
     frames = args.frames or 600
     # the edit stream is input data: drawn for every frame before timing starts
-    kk, jj = np.meshgrid(np.arange(256), np.arange(6), indexing="ij")
-    edit_u = [uniform01(kk, jj, seed=4 + f).tolist()
-              for f in range(frames + args.warmup)]
     frame_edits = []
-    for us in edit_u:
-        fe = []
-        for u in us:
-            r = 3.0 + 21.0 * u[0]
-            ctr = (u[1] * X, u[2] * Y, u[3] * Z)
-            if u[4] < 0.5:
-                fe.append((255, (0, 0, 0), ctr, r, False))
-            else:
-                m = min(n_mats - 1, int(u[5] * n_mats))
-                fe.append((m, sc.materials[m].albedo, ctr, r, False))
+    for f in range(frames + args.warmup):
+        fe = scenes.sphere_edits(sc, f)
         frame_edits.append(Volume.sphere_edits(fe) if args.edit_api == "batch" else fe)
     times, edit_ms, up_bytes, up_chunks, lit = [], [], [], [], []
     vol.counters(KIND_LIGHTING, reset=True)

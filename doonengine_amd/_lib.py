@@ -34,6 +34,10 @@ class DNmat4(Structure):
     _fields_ = [("m", (c_float * 4) * 4)]
 
 
+class DNquaternion(Structure):
+    _fields_ = [("x", c_float), ("y", c_float), ("z", c_float), ("w", c_float)]
+
+
 class DNcolor(Structure):
     _fields_ = [("r", ctypes.c_uint8), ("g", ctypes.c_uint8), ("b", ctypes.c_uint8)]
 
@@ -157,7 +161,7 @@ SIGNATURES = {
     "DN_cam_dir": (DNvec3, [DNvec3]),
     "DN_compress_voxel": (DNcompressedVoxel, [DNvoxel]),
     "DN_decompress_voxel": (DNvoxel, [DNcompressedVoxel]),
-    "DN_shape_sphere": (None, [POINTER(DNvolume), DNvoxel, c_bool, DNvec3, c_float]),
+    "DN_shape_sphere": (None, [POINTER(DNvolume), DNvoxel, c_bool, DNvec3, c_float, DNquaternion, c_void_p, c_void_p]),
     "DNX_struct_size": (c_size_t, [c_int]),
     "DNX_set_message_callback": (None, [MSG_CB]),
     "DNX_error_count": (c_uint32, []),
@@ -168,6 +172,8 @@ SIGNATURES = {
     "DN_load_volume": (POINTER(DNvolume), [ctypes.c_char_p, c_uint32]),
     "DN_save_volume": (c_bool, [ctypes.c_char_p, POINTER(DNvolume)]),
     "DNX_request_all_chunks": (None, [POINTER(DNvolume)]),
+    "DNX_set_voxel_pool_limit": (c_bool, [POINTER(DNvolume), c_size_t]),
+    "DNX_get_residency_stats": (None, [POINTER(DNvolume), POINTER(c_uint64), POINTER(c_uint64), c_bool]),
     "DNX_mark_visible": (c_uint32, [POINTER(DNvolume), c_uint32]),
     "DNX_get_stream": (c_void_p, [POINTER(DNvolume)]),
     "DNX_synchronize": (c_bool, [POINTER(DNvolume)]),

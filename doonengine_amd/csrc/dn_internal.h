@@ -120,6 +120,7 @@ struct DNbatchParams {
     uint32_t unit;                 // requests per work unit, even
     uint32_t queues;               // 1 or 8
     uint32_t dry;                  // 1: compute only, no stores (autotuning)
+    uint32_t* dryLastUsed;         // dry: where rays' lastUsed stores go instead (>= the largest volume's tiles)
     uint32_t* queueCounters;       // [queues] next unit of each queue
     uint32_t* directCounters;      // [queues] next unit of each queue for the direct-ray pass
     const uint32_t* queueScenes;   // [queues][numScenes] volume ids of each queue, in order

@@ -181,6 +181,9 @@ struct DNgpuVolume {
     std::vector<uint8_t> dirtyMark;
     bool fullScan = false;
     bool requestAll = false;
+    bool resetPending = false;         // a failed upload: clear the device tiles and reload everything
+    uint32_t poolLimit = 0;            // voxels; 0: the pool may grow to chunkCap * 512
+    uint64_t evictions = 0, deferrals = 0;
     bool layoutDirty = true;
     // exposed chunks (a solid voxel with an empty or see-through face neighbour),
     // recomputed after edits, and the device list DNX_mark_visible uploads
@@ -307,10 +310,16 @@ bool grow_pool(DNvolume* vol, uint32_t newCap)
     uint4* nv = nullptr;
     if(!dalloc(&nv, (size_t)newCap, "voxel buffer")) return false;
     const size_t old = g->dVoxels ? g->pool.cap : 0;
-    if(old) hipMemcpyAsync(nv, g->dVoxels, sizeof(uint4) * old, hipMemcpyDeviceToDevice, g->stream);
     // free slots read as zero words, so exported pools are deterministic
-    hipMemsetAsync(nv + old, 0, sizeof(uint4) * (newCap - old), g->stream);
-    hipStreamSynchronize(g->stream);
+    if((old && !HIP_OK(hipMemcpyAsync(nv, g->dVoxels, sizeof(uint4) * old, hipMemcpyDeviceToDevice, g->stream),
+                       "voxel buffer copy")) ||
+       !HIP_OK(hipMemsetAsync(nv + old, 0, sizeof(uint4) * (newCap - old), g->stream), "voxel buffer clear") ||
+       !HIP_OK(hipStreamSynchronize(g->stream), "voxel buffer resize"))
+    {
+        hipStreamSynchronize(g->stream);
+        dfree(nv);
+        return false;
+    }
     dfree(g->dVoxels);
     g->dVoxels = nv;
     if(g->dVoxelsOut)
@@ -416,7 +425,13 @@ bool ensure_stage(DNgpuVolume* g, int slot, size_t bytes)
 {
     if(g->hStageCap[slot] < bytes)
     {
-        if(g->hStage[slot]) { hipEventSynchronize(g->stageDone[slot]); hipHostFree(g->hStage[slot]); }
+        if(g->hStage[slot])
+        {
+            hipEventSynchronize(g->stageDone[slot]);
+            hipHostFree(g->hStage[slot]);
+            g->hStage[slot] = nullptr;
+            g->hStageCap[slot] = 0;
+        }
         size_t cap = std::max(bytes, (size_t)1 << 20);
         if(!HIP_OK(hipHostMalloc((void**)&g->hStage[slot], cap, hipHostMallocDefault), "pinned staging"))
         {
@@ -643,22 +658,87 @@ void rebuild_layout(DNvolume* vol)
 
 // Ray-requested tiles (flag 3 written by step_map, voxelShared.comp:462-466).
 // Blocks; only runs while some tile is unloaded.
-void collect_ray_requests(DNvolume* vol)
+// Returns false (after an ERROR message) when a device step fails; the
+// requests then stay flagged on the device and the next sync collects them.
+bool collect_ray_requests(DNvolume* vol)
 {
     DNgpuVolume* g = vol->gpu;
     // after every sync each tile is 0, 1 or 2, so numUnloaded counts state-1 tiles
-    if(g->numUnloaded == 0) return;
+    if(g->numUnloaded == 0) return true;
     uint32_t* dCount = g->dWorkCounter + 1;
-    hipMemsetAsync(dCount, 0, sizeof(uint32_t), g->stream);
-    dn_launch_collect_requested(g->dFlags, g->tiles, g->dReqList, dCount, g->stream);
-    hipMemcpyAsync(g->hSmall, dCount, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream);
-    hipStreamSynchronize(g->stream);
-    uint32_t n = g->hSmall[0];
-    if(n == 0) return;
+    if(!HIP_OK(hipMemsetAsync(dCount, 0, sizeof(uint32_t), g->stream), "ray request count")) return false;
+    int e = dn_launch_collect_requested(g->dFlags, g->tiles, g->dReqList, dCount, g->stream);
+    if(e)
+    {
+        dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR, "ray request collection launch failed (%d)", e);
+        return false;
+    }
+    if(!HIP_OK(hipMemcpyAsync(g->hSmall, dCount, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream),
+               "ray request count readback") ||
+       !HIP_OK(hipStreamSynchronize(g->stream), "ray request collection"))
+        return false;
+    uint32_t n = std::min(g->hSmall[0], g->tiles);
+    if(n == 0) return true;
     std::vector<uint32_t> list(n);
-    hipMemcpy(list.data(), g->dReqList, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
+    if(!HIP_OK(hipMemcpy(list.data(), g->dReqList, sizeof(uint32_t) * n, hipMemcpyDeviceToHost), "ray request readback"))
+        return false;
     for(uint32_t t : list)
         if(t < g->tiles && g->state[t] == 1) { set_state(g, t, 3); mark_dirty(g, t); }
+    return true;
+}
+
+// After a failed upload the device's tile state can no longer be trusted to
+// match the host mirror: the next sync clears every tile's device flags and
+// reloads every CPU chunk from scratch.
+void schedule_device_reset(DNgpuVolume* g)
+{
+    g->resetPending = true;
+}
+
+// Least-recently-used pool node of class >= k among loaded chunks not entered
+// by a ray in the last two syncs (voxel.c:1554-1604: lastUsed > 1, oldest
+// first; among equal ages the smaller node, then the lower map index). Node
+// lists are built once per sync from the device's lastUsed words.
+struct EvictionQueue {
+    std::vector<std::pair<uint32_t, uint32_t>> byClass[6];   // (age, tile), oldest last
+    size_t head[6] = {0, 0, 0, 0, 0, 0};
+    bool ready = false;
+};
+
+bool build_eviction_queue(DNvolume* vol, EvictionQueue& q)
+{
+    DNgpuVolume* g = vol->gpu;
+    q.ready = true;
+    std::vector<uint32_t> ages(g->tiles);
+    if(!HIP_OK(hipMemcpyAsync(ages.data(), g->dLastUsed, sizeof(uint32_t) * g->tiles, hipMemcpyDeviceToHost, g->stream),
+               "lastUsed readback") ||
+       !HIP_OK(hipStreamSynchronize(g->stream), "lastUsed readback"))
+        return false;
+    for(uint32_t t = 0; t < g->tiles; t++)
+        if(g->state[t] == 2 && g->nodeClass[t] >= 0 && g->dirtyMark[t] != 2 && ages[t] > 1u)
+            q.byClass[g->nodeClass[t]].push_back({ages[t], t});
+    for(auto& v : q.byClass)
+        std::sort(v.begin(), v.end(), [](const std::pair<uint32_t, uint32_t>& a, const std::pair<uint32_t, uint32_t>& b) {
+            return a.first != b.first ? a.first > b.first : a.second < b.second;
+        });
+    return true;
+}
+
+// Pops the chunk to evict for a node of class k, or UINT32_MAX.
+uint32_t pop_eviction(DNgpuVolume* g, EvictionQueue& q, int k)
+{
+    int best = -1;
+    for(int j = k; j < 6; j++)
+    {
+        auto& v = q.byClass[j];
+        // skip chunks unloaded since, or loaded by this sync (dirtyMark 2)
+        while(q.head[j] < v.size() && (g->state[v[q.head[j]].second] != 2 || g->dirtyMark[v[q.head[j]].second] == 2))
+            q.head[j]++;
+        if(q.head[j] >= v.size()) continue;
+        if(best < 0 || v[q.head[j]].first > q.byClass[best][q.head[best]].first) best = j;
+    }
+    if(best < 0) return UINT32_MAX;
+    return q.byClass[best][q.head[best]++].second;
 }
 
 }  // namespace
@@ -839,6 +919,235 @@ extern "C" void DN_remove_chunk(DNvolume* vol, DNivec3 pos)
 // SYNC (voxel.c:719-786): streaming state machine on the host mirror, one
 // staged upload, then the device-side visible-chunk compaction.
 
+namespace {
+
+// Streaming half of DN_sync_gpu (voxel.c:1491-1640). Plans the state changes
+// of the dirty tiles (removed chunks dropped, added chunks marked unloaded,
+// updated chunks unloaded and re-requested, requested chunks loaded), stages
+// the upload, and only then commits: if the staging buffer cannot be had,
+// nothing changes and the next sync retries. A chunk to load takes a free pool
+// node, else the node of the least recently used chunk (evicted: its tile goes
+// back to "unloaded", and a ray that enters it requests it again), else the
+// pool grows (doubling, up to chunkCap * 512 voxels and the pool limit), else
+// the load waits for a later request. A device step that fails after the
+// commit schedules a full device reset for the next sync.
+void stream_chunks(DNvolume* vol)
+{
+    DNgpuVolume* g = vol->gpu;
+    if(g->resetPending)
+    {
+        if(!HIP_OK(hipMemsetAsync(g->dFlags, 0, sizeof(uint32_t) * g->tiles, g->stream), "map buffer reset")) return;
+        g->resetPending = false;
+        g->pool.reset(g->pool.cap);
+        for(uint32_t t = 0; t < g->tiles; t++)
+        {
+            g->nodeClass[t] = -1;
+            g->blocks[t] = 0;
+            set_state(g, t, 0);
+        }
+        g->loadedBlocks = 0;
+        g->layoutDirty = true;
+        g->occDirty = true;
+        g->requestAll = true;
+        g->fullScan = true;
+    }
+    collect_ray_requests(vol);
+    if(g->fullScan || g->requestAll)
+    {
+        for(uint32_t t = 0; t < g->tiles; t++)
+            if(g->fullScan || (g->state[t] != 2 && vol->map[t].flag != 0)) mark_dirty(g, t);
+    }
+    std::sort(g->dirty.begin(), g->dirty.end());
+
+    // plan: each dirty tile's next state (voxel.c:1491-1536)
+    struct Plan { uint32_t t; uint8_t before, after; bool freeNode; };
+    std::vector<Plan> plan;
+    std::vector<uint32_t> loads;
+    plan.reserve(g->dirty.size());
+    for(uint32_t t : g->dirty)
+    {
+        const bool cpu = vol->map[t].flag != 0;
+        const DNchunk* ch = cpu ? &vol->chunks[vol->map[t].chunkIndex] : nullptr;
+        uint8_t st = g->state[t];
+        Plan pl = {t, st, st, false};
+        if(cpu && st == 0) st = 1;
+        else if(!cpu && st != 0)
+        {
+            pl.freeNode = st == 2;
+            st = 0;
+        }
+        if(st == 2 && ch && ch->updated) { pl.freeNode = true; st = 3; }
+        if(g->requestAll && cpu && st == 1) st = 3;
+        pl.after = st;
+        if(st == 3 && cpu) loads.push_back(t);
+        plan.push_back(pl);
+    }
+
+    // convert the chunks to load (_DN_chunk_to_gpu, on host threads)
+    std::vector<DNchunkRec> recs(loads.size());
+    std::vector<uint4> voxBuf(loads.size() * 512);
+    std::vector<uint32_t> counts(loads.size());
+    if(!loads.empty())
+    {
+        const DNmaterial* mats = vol->materials;
+        unsigned nth = std::min<unsigned>(8, std::max<unsigned>(1, std::thread::hardware_concurrency()));
+        if(loads.size() < 64) nth = 1;
+        auto work = [&](unsigned w) {
+            for(size_t i = w; i < loads.size(); i += nth)
+            {
+                const DNchunk& ch = vol->chunks[vol->map[loads[i]].chunkIndex];
+                counts[i] = chunk_to_gpu(mats, ch, recs[i], &voxBuf[i * 512]);
+            }
+        };
+        if(nth == 1) work(0);
+        else
+        {
+            std::vector<std::thread> th;
+            for(unsigned w = 0; w < nth; w++) th.emplace_back(work, w);
+            for(auto& x : th) x.join();
+        }
+    }
+    size_t totalVox = 0;
+    for(uint32_t n : counts) totalVox += n;
+    // at most one operation per planned tile plus one eviction per load
+    const size_t maxOps = plan.size() + loads.size();
+    if(maxOps == 0 && !g->occDirty)
+    {
+        g->requestAll = false;
+        g->fullScan = false;
+        return;
+    }
+    const size_t oRec = (sizeof(DNuploadOp) * maxOps + 127) & ~(size_t)127;
+    const size_t oVox = oRec + sizeof(DNchunkRec) * recs.size();
+    const size_t oOcc = oVox + sizeof(uint4) * totalVox;
+    const size_t bytes = oOcc + 2 * sizeof(uint32_t) * g->occWords;
+    const int slot = acquire_stage_slot(g);
+    if(!ensure_stage(g, slot, bytes) || !g->dStage)
+    {
+        dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR, "chunk streaming postponed: no staging memory for %zu bytes",
+               bytes);
+        return;                                  // nothing committed; the dirty tiles stay dirty
+    }
+
+    // commit the plan
+    std::vector<DNuploadOp> ops;
+    ops.reserve(maxOps);
+    for(const Plan& pl : plan)
+    {
+        g->dirtyMark[pl.t] = 0;
+        if(pl.freeNode) free_node(g, pl.t);
+        set_state(g, pl.t, pl.after);
+        if(vol->map[pl.t].flag != 0) vol->chunks[vol->map[pl.t].chunkIndex].updated = false;
+        if(pl.after != pl.before && (pl.after == 0 || pl.after == 1)) ops.push_back(DNuploadOp{pl.t, pl.after, 0, 0});
+    }
+    g->dirty.clear();
+    g->requestAll = false;
+    g->fullScan = false;
+
+    // pool nodes: free node, else evict the least recently used chunk, else grow, else wait.
+    // Chunks loaded by this sync are marked state 3 until their node is found,
+    // then 2, and are never eviction candidates (their device age is stale).
+    EvictionQueue evq;
+    for(uint32_t t : loads) g->dirtyMark[t] = 2;
+    size_t voxOff = 0;
+    for(size_t i = 0; i < loads.size(); i++)
+    {
+        const uint32_t t = loads[i], n = counts[i];
+        vol->chunks[vol->map[t].chunkIndex].numVoxelsGpu = n;
+        const int k = PoolAlloc::cls(n < 16 ? 16 : n);
+        uint32_t start = g->pool.alloc(k);
+        if(start == UINT32_MAX)
+        {
+            if(!evq.ready) build_eviction_queue(vol, evq);
+            uint32_t e = pop_eviction(g, evq, k);
+            if(e != UINT32_MAX)
+            {
+                free_node(g, e);
+                set_state(g, e, 1);
+                ops.push_back(DNuploadOp{e, 1, 0, 0});
+                g->evictions++;
+                start = g->pool.alloc(k);
+            }
+        }
+        if(start == UINT32_MAX)
+        {
+            uint64_t want = std::max<uint64_t>((uint64_t)g->pool.cap * 2, (uint64_t)g->pool.cap + 512);
+            want = std::min<uint64_t>(want, (uint64_t)vol->chunkCap * DN_CHUNK_LENGTH + 512);
+            if(g->poolLimit) want = std::min<uint64_t>(want, g->poolLimit);
+            want = std::min<uint64_t>(want, 0xFFFFFE00ull);
+            if(want > g->pool.cap)
+            {
+                dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_NOTE,
+                       "automatically resizing voxel buffer to accomodate %llu GPU voxels (%llu bytes)",
+                       (unsigned long long)want, (unsigned long long)want * 16ull);
+                if(grow_pool(vol, (uint32_t)want)) start = g->pool.alloc(k);
+            }
+        }
+        if(start == UINT32_MAX)
+        {
+            set_state(g, t, 1);                  // waits for a later request (voxel.c:1607-1611)
+            ops.push_back(DNuploadOp{t, 1, 0, 0});
+            g->deferrals++;
+            continue;
+        }
+        g->nodeStart[t] = start;
+        g->nodeClass[t] = (int8_t)k;
+        g->layoutDirty = true;
+        recs[i].voxelIndex = start;
+        g->blocks[t] = (n + 31u) / 32u;
+        g->loadedBlocks += g->blocks[t];
+        ops.push_back(DNuploadOp{t, 2, (uint32_t)i, (uint32_t)voxOff});
+        set_state(g, t, 2);
+        voxOff += n;
+    }
+
+    for(uint32_t t : loads) g->dirtyMark[t] = 0;
+
+    // stage [ops][records][voxels][occLoaded][occPresent] and upload
+    uint8_t* h = g->hStage[slot];
+    memcpy(h, ops.data(), sizeof(DNuploadOp) * ops.size());
+    if(!recs.empty()) memcpy(h + oRec, recs.data(), sizeof(DNchunkRec) * recs.size());
+    uint4* hv = (uint4*)(h + oVox);
+    for(const DNuploadOp& o : ops)
+        if(o.mode == 2) memcpy(hv + o.voxOffset, &voxBuf[(size_t)o.recSlot * 512], sizeof(uint4) * counts[o.recSlot]);
+    const size_t stagedVox = oVox + sizeof(uint4) * voxOff;
+    memcpy(h + stagedVox, g->occLoaded.data(), sizeof(uint32_t) * g->occWords);
+    memcpy(h + stagedVox + sizeof(uint32_t) * g->occWords, g->occPresent.data(), sizeof(uint32_t) * g->occWords);
+    const size_t used = stagedVox + 2 * sizeof(uint32_t) * g->occWords;
+    Timer tm = timer_begin(g, TK_UPLOAD);
+    bool ok = HIP_OK(hipMemcpyAsync(g->dStage, h, used, hipMemcpyHostToDevice, g->stream), "chunk upload") &&
+              HIP_OK(hipMemcpyAsync(g->dOccLoaded, g->dStage + stagedVox, sizeof(uint32_t) * g->occWords,
+                                    hipMemcpyDeviceToDevice, g->stream), "occupancy upload") &&
+              HIP_OK(hipMemcpyAsync(g->dOccPresent, g->dStage + stagedVox + sizeof(uint32_t) * g->occWords,
+                                    sizeof(uint32_t) * g->occWords, hipMemcpyDeviceToDevice, g->stream),
+                     "occupancy upload");
+    if(ok && !ops.empty())
+    {
+        int e = dn_launch_upload(g->dFlags, g->dLastUsed, g->dChunks, g->dVoxels, g->dFresh,
+                                 (const DNuploadOp*)g->dStage, (uint32_t)ops.size(), (const DNchunkRec*)(g->dStage + oRec),
+                                 (const uint4*)(g->dStage + oVox), g->stream);
+        if(e)
+        {
+            dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR, "chunk upload launch failed (%d)", e);
+            ok = false;
+        }
+    }
+    timer_end(g, tm);
+    release_stage_slot(g, slot);
+    g->upBytes += used;
+    g->upChunks += recs.size();
+    g->upVoxels += voxOff;
+    if(!ok)
+    {
+        schedule_device_reset(g);
+        return;
+    }
+    g->occDirty = false;
+    rebuild_layout(vol);
+}
+
+}  // namespace
+
 extern "C" void DN_sync_gpu(DNvolume* vol, DNmemOp op, int lightingSplit)
 {
     DNgpuVolume* g = vol->gpu;
@@ -850,162 +1159,7 @@ extern "C" void DN_sync_gpu(DNvolume* vol, DNmemOp op, int lightingSplit)
     vol->frameNum++;
     if(vol->frameNum >= (uint32_t)lightingSplit) vol->frameNum = 0;
 
-    if(op != DN_READ)
-    {
-        collect_ray_requests(vol);
-        if(g->fullScan || g->requestAll)
-        {
-            for(uint32_t t = 0; t < g->tiles; t++)
-                if(g->fullScan || (g->state[t] != 2 && vol->map[t].flag != 0)) mark_dirty(g, t);
-            g->fullScan = false;
-        }
-        std::sort(g->dirty.begin(), g->dirty.end());
-
-        // state transitions (voxel.c:1491-1536) and the list of chunks to load
-        std::vector<DNuploadOp> ops;
-        std::vector<uint32_t> loads;
-        ops.reserve(g->dirty.size());
-        for(uint32_t t : g->dirty)
-        {
-            g->dirtyMark[t] = 0;
-            bool cpu = vol->map[t].flag != 0;
-            DNchunk* ch = cpu ? &vol->chunks[vol->map[t].chunkIndex] : nullptr;
-            uint8_t st = g->state[t];
-            uint8_t before = st;
-            if(cpu && st == 0) st = 1;
-            else if(!cpu && st != 0)
-            {
-                if(st == 2) free_node(g, t);
-                st = 0;
-            }
-            if(st == 2 && ch && ch->updated) { free_node(g, t); st = 3; }
-            if(g->requestAll && cpu && st == 1) st = 3;
-            if(st == 3 && cpu) loads.push_back(t);
-            else if(st != before && (st == 0 || st == 1))
-            {
-                DNuploadOp o = {t, st, 0, 0};
-                ops.push_back(o);
-            }
-            set_state(g, t, st);
-            if(ch) ch->updated = false;
-        }
-        g->dirty.clear();
-        g->requestAll = false;
-
-        // convert, allocate and stage the chunks to load
-        if(!loads.empty())
-        {
-            std::vector<DNchunkRec> recs(loads.size());
-            std::vector<uint4> voxBuf(loads.size() * 512);
-            std::vector<uint32_t> counts(loads.size());
-            const DNmaterial* mats = vol->materials;
-            unsigned nth = std::min<unsigned>(8, std::max<unsigned>(1, std::thread::hardware_concurrency()));
-            if(loads.size() < 64) nth = 1;
-            auto work = [&](unsigned w) {
-                for(size_t i = w; i < loads.size(); i += nth)
-                {
-                    const DNchunk& ch = vol->chunks[vol->map[loads[i]].chunkIndex];
-                    counts[i] = chunk_to_gpu(mats, ch, recs[i], &voxBuf[i * 512]);
-                }
-            };
-            if(nth == 1) work(0);
-            else
-            {
-                std::vector<std::thread> th;
-                for(unsigned w = 0; w < nth; w++) th.emplace_back(work, w);
-                for(auto& x : th) x.join();
-            }
-            size_t totalVox = 0;
-            for(size_t i = 0; i < loads.size(); i++)
-            {
-                uint32_t t = loads[i];
-                uint32_t n = counts[i];
-                vol->chunks[vol->map[t].chunkIndex].numVoxelsGpu = n;
-                int k = PoolAlloc::cls(n < 16 ? 16 : n);
-                uint32_t start = g->pool.alloc(k);
-                if(start == UINT32_MAX)
-                {
-                    uint32_t want = std::max(g->pool.cap * 2, g->pool.cap + 512u);
-                    dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_NOTE,
-                           "automatically resizing voxel buffer to accomodate %u GPU voxels (%zu bytes)", want,
-                           (size_t)want * 16);
-                    if(!grow_pool(vol, want) || (start = g->pool.alloc(k)) == UINT32_MAX)
-                    {
-                        set_state(g, t, 1);
-                        DNuploadOp o = {t, 1, 0, 0};
-                        ops.push_back(o);
-                        continue;
-                    }
-                }
-                g->nodeStart[t] = start;
-                g->nodeClass[t] = (int8_t)k;
-                g->layoutDirty = true;
-                recs[i].voxelIndex = start;
-                g->blocks[t] = (n + 31u) / 32u;
-                g->loadedBlocks += g->blocks[t];
-                DNuploadOp o = {t, 2, (uint32_t)i, (uint32_t)totalVox};
-                ops.push_back(o);
-                set_state(g, t, 2);
-                totalVox += n;
-            }
-            // stage: [ops][records][voxels][occLoaded][occPresent]
-            size_t oOps = 0, oRec = (sizeof(DNuploadOp) * ops.size() + 127) & ~(size_t)127;
-            size_t oVox = oRec + sizeof(DNchunkRec) * recs.size();
-            size_t oOcc = oVox + sizeof(uint4) * totalVox;
-            size_t bytes = oOcc + 2 * sizeof(uint32_t) * g->occWords;
-            int s = acquire_stage_slot(g);
-            if(ensure_stage(g, s, bytes))
-            {
-                uint8_t* h = g->hStage[s];
-                memcpy(h + oOps, ops.data(), sizeof(DNuploadOp) * ops.size());
-                memcpy(h + oRec, recs.data(), sizeof(DNchunkRec) * recs.size());
-                uint4* hv = (uint4*)(h + oVox);
-                for(const DNuploadOp& o : ops)
-                    if(o.mode == 2) memcpy(hv + o.voxOffset, &voxBuf[(size_t)o.recSlot * 512], sizeof(uint4) * counts[o.recSlot]);
-                memcpy(h + oOcc, g->occLoaded.data(), sizeof(uint32_t) * g->occWords);
-                memcpy(h + oOcc + sizeof(uint32_t) * g->occWords, g->occPresent.data(), sizeof(uint32_t) * g->occWords);
-                Timer tm = timer_begin(g, TK_UPLOAD);
-                hipMemcpyAsync(g->dStage, h, bytes, hipMemcpyHostToDevice, g->stream);
-                g->upBytes += bytes;
-                g->upChunks += recs.size();
-                g->upVoxels += totalVox;
-                hipMemcpyAsync(g->dOccLoaded, g->dStage + oOcc, sizeof(uint32_t) * g->occWords, hipMemcpyDeviceToDevice,
-                               g->stream);
-                hipMemcpyAsync(g->dOccPresent, g->dStage + oOcc + sizeof(uint32_t) * g->occWords,
-                               sizeof(uint32_t) * g->occWords, hipMemcpyDeviceToDevice, g->stream);
-                dn_launch_upload(g->dFlags, g->dLastUsed, g->dChunks, g->dVoxels, g->dFresh,
-                                 (const DNuploadOp*)(g->dStage + oOps), (uint32_t)ops.size(),
-                                 (const DNchunkRec*)(g->dStage + oRec), (const uint4*)(g->dStage + oVox), g->stream);
-                timer_end(g, tm);
-                release_stage_slot(g, s);
-                g->occDirty = false;
-            }
-        }
-        else if(!ops.empty() || g->occDirty)
-        {
-            size_t oOcc = (sizeof(DNuploadOp) * ops.size() + 127) & ~(size_t)127;
-            size_t bytes = oOcc + 2 * sizeof(uint32_t) * g->occWords;
-            int s = acquire_stage_slot(g);
-            if(ensure_stage(g, s, bytes))
-            {
-                uint8_t* h = g->hStage[s];
-                memcpy(h, ops.data(), sizeof(DNuploadOp) * ops.size());
-                memcpy(h + oOcc, g->occLoaded.data(), sizeof(uint32_t) * g->occWords);
-                memcpy(h + oOcc + sizeof(uint32_t) * g->occWords, g->occPresent.data(), sizeof(uint32_t) * g->occWords);
-                hipMemcpyAsync(g->dStage, h, bytes, hipMemcpyHostToDevice, g->stream);
-                g->upBytes += bytes;
-                hipMemcpyAsync(g->dOccLoaded, g->dStage + oOcc, sizeof(uint32_t) * g->occWords, hipMemcpyDeviceToDevice,
-                               g->stream);
-                hipMemcpyAsync(g->dOccPresent, g->dStage + oOcc + sizeof(uint32_t) * g->occWords,
-                               sizeof(uint32_t) * g->occWords, hipMemcpyDeviceToDevice, g->stream);
-                dn_launch_upload(g->dFlags, g->dLastUsed, g->dChunks, g->dVoxels, g->dFresh,
-                                 (const DNuploadOp*)g->dStage, (uint32_t)ops.size(), nullptr, nullptr, g->stream);
-                release_stage_slot(g, s);
-                g->occDirty = false;
-            }
-        }
-        rebuild_layout(vol);
-    }
+    if(op != DN_READ) stream_chunks(vol);
 
     DNvisibleParams vp;
     vp.flags = g->dFlags;
@@ -1365,13 +1519,36 @@ extern "C" bool DN_set_max_chunks(DNvolume* vol, size_t num)
 
 extern "C" bool DN_set_max_voxels_gpu(DNvolume* vol, size_t num)
 {
-    if(num < vol->gpu->pool.cap || num > 0xFFFFFE00u)
+    if(num < vol->gpu->pool.cap || num > 0xFFFFFE00u || (vol->gpu->poolLimit && num > vol->gpu->poolLimit))
     {
         dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR, "the voxel buffer can only grow (requested %zu, have %u)", num,
                vol->gpu->pool.cap);
         return false;
     }
     return grow_pool(vol, (uint32_t)num);
+}
+
+extern "C" bool DNX_set_voxel_pool_limit(DNvolume* vol, size_t maxVoxels)
+{
+    DNgpuVolume* g = vol->gpu;
+    if(maxVoxels == 0) { g->poolLimit = 0; return true; }
+    size_t lim = (maxVoxels + 511u) & ~(size_t)511u;
+    if(lim < g->pool.cap || lim > 0xFFFFFE00u)
+    {
+        dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR, "voxel pool limit %zu is below the pool's %u voxels or too large",
+               maxVoxels, g->pool.cap);
+        return false;
+    }
+    g->poolLimit = (uint32_t)lim;
+    return true;
+}
+
+extern "C" void DNX_get_residency_stats(DNvolume* vol, uint64_t* evictions, uint64_t* deferrals, bool reset)
+{
+    DNgpuVolume* g = vol->gpu;
+    if(evictions) *evictions = g->evictions;
+    if(deferrals) *deferrals = g->deferrals;
+    if(reset) g->evictions = g->deferrals = 0;
 }
 
 extern "C" bool DN_set_max_lighting_requests(DNvolume* vol, size_t num)
@@ -1608,9 +1785,41 @@ inline float nearest_offset(int base, float cc)
 }
 }  // namespace
 
-// voxelShapes.c:239-333 without the transform callback.
-extern "C" void DN_shape_sphere(DNvolume* vol, DNvoxel voxel, bool flipNormals, DNvec3 c, float r)
+namespace {
+// QuickMath's quaternion_to_mat4 (quickmath.h:1622-1650): rotation part of a
+// unit quaternion, written into an identity matrix (column-major m[col][row]).
+DNmat4 quaternion_mat(DNquaternion q)
 {
+    DNmat4 r = mat_identity();
+    float x2 = q.x + q.x, y2 = q.y + q.y, z2 = q.z + q.z;
+    float xx2 = q.x * x2, xy2 = q.x * y2, xz2 = q.x * z2;
+    float yy2 = q.y * y2, yz2 = q.y * z2, zz2 = q.z * z2;
+    float sx2 = q.w * x2, sy2 = q.w * y2, sz2 = q.w * z2;
+    r.m[0][0] = 1.0f - (yy2 + zz2); r.m[0][1] = xy2 - sz2;          r.m[0][2] = xz2 + sy2;
+    r.m[1][0] = xy2 + sz2;          r.m[1][1] = 1.0f - (xx2 + zz2); r.m[1][2] = yz2 - sx2;
+    r.m[2][0] = xz2 - sy2;          r.m[2][1] = yz2 + sx2;          r.m[2][2] = 1.0f - (xx2 + yy2);
+    return r;
+}
+}  // namespace
+
+// voxelShapes.c:239-333. Visits the sphere's tile box in z, y, x order and each
+// tile's voxels in z, y, x order; the tile's chunk-present flag is read once
+// per tile, as the reference copies the map tile before its voxel loop. With
+// `func`, the voxel placed (and a flipped rim voxel) is what func returns for
+// the voxel's position, its computed normal, the voxel (the argument, or the
+// old voxel on the rim), the sphere's bounding box and the inverse of
+// translate(c) * rotation(orient) (voxelShapes.c:242-246, 297-303, 316-321).
+extern "C" void DN_shape_sphere(DNvolume* vol, DNvoxel voxel, bool flipNormals, DNvec3 c, float r, DNquaternion orient,
+                                VoxelTransformFunc func, void* userData)
+{
+    DNmat4 invTransform = mat_identity();
+    if(func)
+    {
+        DNmat4 translate = mat_identity();
+        translate.m[3][0] = c.x; translate.m[3][1] = c.y; translate.m[3][2] = c.z;
+        invTransform = mat_inv(mat_mult(translate, quaternion_mat(orient)));
+    }
+    const DNvec3 shapeMin = v3(c.x - r, c.y - r, c.z - r), shapeMax = v3(c.x + r, c.y + r, c.z + r);
     const SphereBox b = sphere_box(vol, voxel.material, flipNormals, c, r);
     const DNivec3 mapMin = b.lo, mapMax = b.hi;
     const float r2 = b.r2, r12 = b.r12;
@@ -1622,6 +1831,7 @@ extern "C" void DN_shape_sphere(DNvolume* vol, DNvoxel voxel, bool flipNormals, 
         DNivec3 mp = {mX, mY, mZ};
         const float nx = nearest_offset(mX * 8, c.x), ny = nearest_offset(mY * 8, c.y), nz = nearest_offset(mZ * 8, c.z);
         if(!touched(nx * nx + ny * ny + nz * nz)) continue;
+        const bool tileExists = vol->map[flat(vol, mp)].flag != 0;
         for(int cZ = 0; cZ < 8; cZ++)
         for(int cY = 0; cY < 8; cY++)
         {
@@ -1633,7 +1843,6 @@ extern "C" void DN_shape_sphere(DNvolume* vol, DNvoxel voxel, bool flipNormals, 
             DNvec3 pos = v3((float)(mX * 8 + cX), (float)(mY * 8 + cY), (float)(mZ * 8 + cZ));
             DNvec3 fc = v3(pos.x - c.x, pos.y - c.y, pos.z - c.z);
             float d2 = fc.x * fc.x + fc.y * fc.y + fc.z * fc.z;
-            bool tileExists = vol->map[flat(vol, mp)].flag != 0;
             if(d2 < r2)
             {
                 if(tileExists && DN_does_voxel_exist(vol, mp, cp) && voxel.material != DN_MATERIAL_EMPTY) continue;
@@ -1642,7 +1851,9 @@ extern "C" void DN_shape_sphere(DNvolume* vol, DNvoxel voxel, bool flipNormals, 
                 {
                     float mxn = fmaxf(fmaxf(fabsf(fc.x), fabsf(fc.y)), fabsf(fc.z));
                     float s = 1.0f / mxn;
-                    fin.normal = v3(fc.x * s, fc.y * s, fc.z * s);
+                    DNvec3 n = v3(fc.x * s, fc.y * s, fc.z * s);
+                    if(func) fin = func(pos, n, voxel, shapeMin, shapeMax, invTransform, userData);
+                    else fin.normal = n;
                 }
                 DN_set_voxel(vol, mp, cp, fin);
             }
@@ -1652,7 +1863,9 @@ extern "C" void DN_shape_sphere(DNvolume* vol, DNvoxel voxel, bool flipNormals, 
                 DNvoxel old = DN_get_voxel(vol, mp, cp);
                 float mxn = fmaxf(fmaxf(fabsf(fc.x), fabsf(fc.y)), fabsf(fc.z));
                 float s = -1.0f / mxn;
-                old.normal = v3(fc.x * s, fc.y * s, fc.z * s);
+                DNvec3 n = v3(fc.x * s, fc.y * s, fc.z * s);
+                if(func) old = func(pos, n, old, shapeMin, shapeMax, invTransform, userData);
+                else old.normal = n;
                 DN_set_voxel(vol, mp, cp, old);
             }
         }
@@ -2549,6 +2762,7 @@ DNbatchParams batch_params(DNXbatch* b, bool dry)
     bp.queueScenes = b->dQueueScenes;
     bp.queueLen = b->dQueueLen;
     bp.queuePrefix = b->dQueuePrefix;
+    bp.dryLastUsed = nullptr;
     return bp;
 }
 
@@ -2753,6 +2967,8 @@ extern "C" bool DNX_batch_autotune(DNXbatch* b, int numDiffuseSamples, int maxDi
     if(!HIP_OK(hipMemcpyAsync(b->dParams, hp, sizeof(DNlightParams) * b->vols.size(), hipMemcpyHostToDevice,
                               b->stream), "batch parameters"))
         return false;
+    uint32_t* scratchAges = nullptr;            // the dry passes' lastUsed stores
+    if(!dalloc(&scratchAges, std::max<uint32_t>(maxTiles, 1u), "autotune scratch")) return false;
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
@@ -2772,7 +2988,9 @@ extern "C" bool DNX_batch_autotune(DNXbatch* b, int numDiffuseSamples, int maxDi
                 for(int r = 0; r < repeats; r++)
                 {
                     hipEventRecord(e0, b->stream);
-                    dn_launch_batch(batch_params(b, true), th, 0, maxTiles, maxReq, false, b->anyDirect, b->stream);
+                    DNbatchParams bp = batch_params(b, true);
+                    bp.dryLastUsed = scratchAges;
+                    dn_launch_batch(bp, th, 0, maxTiles, maxReq, false, b->anyDirect, b->stream);
                     hipEventRecord(e1, b->stream);
                     hipEventSynchronize(e1);
                     hipEventElapsedTime(&ms, e0, e1);
@@ -2782,6 +3000,8 @@ extern "C" bool DNX_batch_autotune(DNXbatch* b, int numDiffuseSamples, int maxDi
             }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
+    hipStreamSynchronize(b->stream);
+    dfree(scratchAges);
     b->cfg = ok ? win : keep;
     ok = ok && batch_upload_queues(b);
     hipEventRecord(b->ready, b->stream);

@@ -397,7 +397,14 @@ void k_direct_batch(DNbatchParams B, const DNlightParams* __restrict__ scenes)
             const uint32_t sid = __builtin_amdgcn_readfirstlane(B.queueScenes[(size_t)q * B.numScenes + lo]);
             const DNlightParams& P = scenes[sid];
             if(!P.directOut) continue;
-            const DNsceneView sv = P.scene;
+            // The direct terms are per-frame scratch of the pass, written in a dry
+            // run too (the dry lighting pass reads them); volume state is not.
+            DNsceneView sv = P.scene;
+            if(B.dry)
+            {
+                sv.hasUnloaded = 0u;
+                sv.lastUsed = B.dryLastUsed;
+            }
             const uint32_t N = P.numDirectRays;
             const uint32_t per = 64u / N, v = lane / N, k = lane - v * N;
             const f3 dir = ld3(P.directRays[k]), inv = ld3(P.directRays[k] + 3), sun = ld3(sv.sunStrength);
@@ -720,7 +727,13 @@ void k_lighting_batch(DNbatchParams B, const DNlightParams* __restrict__ scenes)
                 cur = sid;
             }
             const DNlightParams& P = scenes[sid];
-            const DNsceneView sv = P.scene;             // once per unit, into registers
+            DNsceneView sv = P.scene;                   // once per unit, into registers
+            if(dry)
+            {
+                // compute only: no chunk requests, lastUsed stores into scratch
+                sv.hasUnloaded = 0u;
+                sv.lastUsed = B.dryLastUsed;
+            }
             const uint32_t nreq = *P.numRequests;
             const uint32_t r0 = (u - pre[lo]) * B.unit;
             for(uint32_t j = 0; j < B.unit; j += 2u)

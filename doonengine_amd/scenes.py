@@ -272,3 +272,24 @@ def batch_scene(index: int, max_grid: int = 48) -> Scene:
     params = {"numSpecularRays": 16, "numDirectRays": 16, "diffuseBounceLimit": 4}
     return Scene(f"batch{index}", (gx, gy, gz), nw, aw, mats, cam_pos=(gx * 0.5, gy * 0.9, gz * 0.5),
                  cam_orient=(45.0, 45.0, 0.0), params=params, visible=visible)
+
+
+def sphere_edits(scene: Scene, frame: int, n: int = 256) -> list:
+    """The dynamic-edit workload's edits of one frame: `n` spheres of radius
+    U[3, 24] voxels, centre uniform over the map, carve or fill 50/50 (a fill
+    takes one of the scene's non-emissive materials and its albedo), seed
+    4 + frame. Each edit is (material, albedo, centre, radius, flip_normals)."""
+    X, Y, Z = (8 * c for c in scene.map_size)
+    n_mats = len(scene.materials) - 1
+    kk, jj = np.meshgrid(np.arange(n), np.arange(6), indexing="ij")
+    us = uniform01(kk, jj, seed=4 + frame).tolist()
+    out = []
+    for u in us:
+        r = 3.0 + 21.0 * u[0]
+        ctr = (u[1] * X, u[2] * Y, u[3] * Z)
+        if u[4] < 0.5:
+            out.append((255, (0, 0, 0), ctr, r, False))
+        else:
+            m = min(n_mats - 1, int(u[5] * n_mats))
+            out.append((m, scene.materials[m].albedo, ctr, r, False))
+    return out

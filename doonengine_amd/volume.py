@@ -133,9 +133,16 @@ class Volume:
     def remove_voxel(self, map_pos, chunk_pos) -> None:
         self.lib.DN_remove_voxel(self.ptr, DNivec3(*map_pos), DNivec3(*chunk_pos))
 
-    def shape_sphere(self, material: int, albedo, center, radius: float, flip_normals=False) -> None:
+    def shape_sphere(self, material: int, albedo, center, radius: float, flip_normals=False,
+                     orient=(0.0, 0.0, 0.0, 1.0), func=None, user_data=None) -> None:
+        """DN_shape_sphere; `func` is None or the address of a C VoxelTransformFunc
+        (voxelShapes.h:28), `user_data` None or an address handed to it."""
         vox = _lib.DNvoxel(material, DNvec3(0, 0, 0), _lib.DNcolor(*albedo))
-        self.lib.DN_shape_sphere(self.ptr, vox, bool(flip_normals), DNvec3(*center), float(radius))
+        fp = None if func is None else ctypes.c_void_p(func if isinstance(func, int) else
+                                                       ctypes.cast(func, ctypes.c_void_p).value)
+        ud = None if user_data is None else ctypes.c_void_p(user_data)
+        self.lib.DN_shape_sphere(self.ptr, vox, bool(flip_normals), DNvec3(*center), float(radius),
+                                 _lib.DNquaternion(*orient), fp, ud)
 
     @staticmethod
     def sphere_edits(edits):
@@ -196,6 +203,14 @@ class Volume:
 
     def request_all(self) -> None:
         self.lib.DNX_request_all_chunks(self.ptr)
+
+    def set_pool_limit(self, max_voxels: int) -> bool:
+        return bool(self.lib.DNX_set_voxel_pool_limit(self.ptr, int(max_voxels)))
+
+    def residency_stats(self, reset: bool = False) -> dict:
+        e, d = ctypes.c_uint64(), ctypes.c_uint64()
+        self.lib.DNX_get_residency_stats(self.ptr, ctypes.byref(e), ctypes.byref(d), bool(reset))
+        return {"evictions": e.value, "deferrals": d.value}
 
     def mark_visible(self, max_chunks: int) -> int:
         return int(self.lib.DNX_mark_visible(self.ptr, int(max_chunks)))

@@ -44,6 +44,13 @@ typedef struct DNvec3 { float x, y, z; } DNvec3;
 typedef struct DNvec4 { float x, y, z, w; } DNvec4;
 typedef struct DNmat4 { float m[4][4]; } DNmat4;
 typedef struct DNquaternion { float x, y, z, w; } DNquaternion;
+/* QuickMath's DN_quaternion_identity (quickmath.h:1338-1348, QM_PREFIX DN_), which
+ * the reference's callers pass as the orientation of the shape functions. */
+static inline DNquaternion DN_quaternion_identity(void)
+{
+    DNquaternion q = {0.0f, 0.0f, 0.0f, 1.0f};
+    return q;
+}
 
 /* ---------------------------------------------------------------------------
  * globals.h:45-63 -- message callback
@@ -227,10 +234,21 @@ DNvec3 DN_cam_dir(DNvec3 orient);                                      /* voxel.
 DNcompressedVoxel DN_compress_voxel(DNvoxel voxel);                    /* voxel.h:366 */
 DNvoxel DN_decompress_voxel(DNcompressedVoxel voxel);                  /* voxel.h:371 */
 
-/* voxelShapes.h:40 (DN_shape_sphere without the transform callback): sets every
- * voxel within radius r of c to `voxel` (normals point away from c), or carves
- * them when voxel.material == DN_MATERIAL_EMPTY; flipNormals re-points the rim. */
-void DN_shape_sphere(DNvolume* vol, DNvoxel voxel, bool flipNormals, DNvec3 c, float r);
+/* voxelShapes.h:19-28: the transform callback of the shape functions. Called
+ * with the voxel's position (in voxels), its computed normal, the voxel given to
+ * the shape (or, on a carve's rim, the voxel already there), the shape's
+ * bounding box and the inverse of its transform; returns the voxel to place. */
+typedef DNvoxel (*VoxelTransformFunc)(DNvec3 pos, DNvec3 normal, DNvoxel vox, DNvec3 shapeMin, DNvec3 shapeMax,
+                                      DNmat4 invTransform, void* userData);
+
+/* voxelShapes.h:40: sets every voxel within radius r of c to `voxel` (normals
+ * point away from c), or carves them when voxel.material == DN_MATERIAL_EMPTY;
+ * flipNormals re-points the rim's normals inwards. orient ({x, y, z, w}
+ * quaternion) and userData only reach func; func NULL places `voxel` with the
+ * computed normal. Runs on the host against the CPU chunks, like every edit;
+ * DNX_shape_spheres is the batched device path for func == NULL. */
+void DN_shape_sphere(DNvolume* vol, DNvoxel voxel, bool flipNormals, DNvec3 c, float r, DNquaternion orient,
+                     VoxelTransformFunc func, void* userData);
 
 /* ---------------------------------------------------------------------------
  * DNX_* -- extensions (no counterpart in the reference)
@@ -283,6 +301,16 @@ void DNX_get_voxels_dense(DNvolume* vol, DNivec3 origin, DNuvec3 size, uint32_t*
 /* Residency: marks every chunk present on the CPU to be (re)loaded by the next
  * DN_sync_gpu, instead of waiting for rays to request it. */
 void DNX_request_all_chunks(DNvolume* vol);
+
+/* Bounds the GPU voxel pool at maxVoxels (rounded up to 512; 0 removes the
+ * bound). A chunk to load takes a free pool node, else the node of the least
+ * recently used chunk not entered by a ray in the last two syncs (voxel.c
+ * :1554-1611), else the pool doubles up to this bound (and up to chunkCap *
+ * 512 voxels, as the reference's resize), else its load waits for a later
+ * request. Fails (false, message) when the pool is already larger. */
+bool DNX_set_voxel_pool_limit(DNvolume* vol, size_t maxVoxels);
+/* Chunks evicted to make room, and loads that had to wait, since the last reset. */
+void DNX_get_residency_stats(DNvolume* vol, uint64_t* evictions, uint64_t* deferrals, bool reset);
 
 /* Marks up to maxChunks exposed loaded chunks (map-index order) visible, as
  * primary rays hitting them would (voxelDraw.comp:118-121): the next DN_sync_gpu

@@ -69,17 +69,24 @@ class EditGrid:
     def set_dense(self, normal_words: np.ndarray, albedo_words: np.ndarray) -> None:
         """DNX_set_voxels_dense from the origin: voxel by voxel, x fastest."""
         zs, ys, xs = normal_words.shape
-        for z in range(zs):
-            for y in range(ys):
-                for x in range(xs):
-                    self.set_word(x, y, z, int(normal_words[z, y, x]), int(albedo_words[z, y, x]))
+        n = np.ascontiguousarray(normal_words, np.uint32)
+        a = np.ascontiguousarray(albedo_words, np.uint32)
+        self.o.lib.or_set_dense(ctypes.byref(self.g), 0, 0, 0, xs, ys, zs, n.ctypes.data_as(c_void_p),
+                                a.ctypes.data_as(c_void_p))
 
-    def shape_sphere(self, material: int, albedo, center, radius: float, flip_normals=False, normal=(0, 0, 0)):
+    def shape_sphere(self, material: int, albedo, center, radius: float, flip_normals=False, normal=(0, 0, 0),
+                     orient=None, fn=None, user_data=None):
+        """DN_shape_sphere; `fn` is the address of a C VoxelTransformFunc (or None),
+        called with `user_data` (an address) and the inverse of
+        translate(center) * rotation(orient), orient an (x, y, z, w) quaternion."""
         n = np.asarray(normal, np.float32)
         word = (int(albedo[0]) << 24) | (int(albedo[1]) << 16) | (int(albedo[2]) << 8)
-        self.o.lib.or_shape_sphere(ctypes.byref(self.g), c_uint32(material), n.ctypes.data_as(c_void_p),
-                                   c_uint32(word), c_int(bool(flip_normals)), c_float(center[0]),
-                                   c_float(center[1]), c_float(center[2]), c_float(radius))
+        q = None if orient is None else np.ascontiguousarray(orient, np.float32)
+        self.o.lib.or_shape_sphere_fn(ctypes.byref(self.g), c_uint32(material), n.ctypes.data_as(c_void_p),
+                                      c_uint32(word), c_int(bool(flip_normals)), c_float(center[0]),
+                                      c_float(center[1]), c_float(center[2]), c_float(radius),
+                                      None if q is None else q.ctypes.data_as(c_void_p),
+                                      c_void_p(fn) if fn else None, c_void_p(user_data) if user_data else None)
 
 
 RAYHIT_DTYPE = np.dtype([("hit", "<u4"), ("voxelIndex", "<u4"), ("mapSteps", "<u4"), ("voxelSteps", "<u4"),
@@ -217,8 +224,9 @@ class Oracle:
         return out, {f: getattr(cnt, f) for f, _ in cnt._fields_}
 
     def draw(self, scene: ORscene, inv_view: np.ndarray, inv_view_proj: np.ndarray, width: int, height: int,
-             view_mode: int = 0, flags_out: np.ndarray | None = None):
-        """inv_view / inv_view_proj: (4,4) float32 indexed [col, row]."""
+             view_mode: int = 0, flags_out: np.ndarray | None = None, threads: int = 1):
+        """inv_view / inv_view_proj: (4,4) float32 indexed [col, row]. threads > 1
+        splits the rows over host threads (same result)."""
         dp = ORdrawParams()
         dp.invView[:] = [float(v) for v in np.asarray(inv_view, np.float32).reshape(-1)]
         dp.invViewProj[:] = [float(v) for v in np.asarray(inv_view_proj, np.float32).reshape(-1)]
@@ -228,15 +236,20 @@ class Oracle:
         cnt = ORcounters()
         fo = None if flags_out is None else flags_out.ctypes.data_as(c_void_p)
         t0 = _time.perf_counter()
-        self.lib.or_draw(ctypes.byref(scene), ctypes.byref(dp), rgba.ctypes.data_as(c_void_p),
-                         px.ctypes.data_as(c_void_p), fo, ctypes.byref(cnt))
+        if threads > 1:
+            self.lib.or_draw_mt(ctypes.byref(scene), ctypes.byref(dp), rgba.ctypes.data_as(c_void_p),
+                                px.ctypes.data_as(c_void_p), fo, ctypes.byref(cnt), c_uint32(threads))
+        else:
+            self.lib.or_draw(ctypes.byref(scene), ctypes.byref(dp), rgba.ctypes.data_as(c_void_p),
+                             px.ctypes.data_as(c_void_p), fo, ctypes.byref(cnt))
         dt = _time.perf_counter() - t0
         return rgba, px, dict({f: getattr(cnt, f) for f, _ in cnt._fields_}, seconds=dt)
 
     def lighting(self, scene: ORscene, requests: np.ndarray, *, time: float, num_diffuse=1, max_diffuse=1000,
                  diffuse_bounces=5, spec_bounces=2, sun_dir=(1, 1, 1), ambient=(0.01, 0.01, 0.01), softness=10.0,
-                 cam_pos=(0, 0, 0), spec_points: np.ndarray, direct_points: np.ndarray | None = None):
-        """One lighting dispatch. Returns (voxels_out, map_out, chunks_out, prequant, vox_of, counters)."""
+                 cam_pos=(0, 0, 0), spec_points: np.ndarray, direct_points: np.ndarray | None = None, threads: int = 1):
+        """One lighting dispatch. Returns (voxels_out, map_out, chunks_out, prequant, vox_of, counters).
+        threads > 1 splits the requests over host threads (same result)."""
         s = scene
         m_in, c_in, v_in = s._keep[0], s._keep[1], s._keep[2]
         lp = ORlightParams()
@@ -264,10 +277,10 @@ class Oracle:
         vox_of = np.zeros(R * 32, np.uint32)
         cnt = ORcounters()
         t0 = _time.perf_counter()
-        self.lib.or_lighting(ctypes.byref(s), ctypes.byref(lp), req.ctypes.data_as(c_void_p), c_uint32(len(req)),
-                             v_out.ctypes.data_as(c_void_p), m_out.ctypes.data_as(c_void_p),
-                             c_out.ctypes.data_as(c_void_p), preq.ctypes.data_as(c_void_p),
-                             vox_of.ctypes.data_as(c_void_p), ctypes.byref(cnt))
+        self.lib.or_lighting_mt(ctypes.byref(s), ctypes.byref(lp), req.ctypes.data_as(c_void_p), c_uint32(len(req)),
+                                v_out.ctypes.data_as(c_void_p), m_out.ctypes.data_as(c_void_p),
+                                c_out.ctypes.data_as(c_void_p), preq.ctypes.data_as(c_void_p),
+                                vox_of.ctypes.data_as(c_void_p), ctypes.byref(cnt), c_uint32(max(1, int(threads))))
         dt = _time.perf_counter() - t0
         return v_out, m_out, c_out, preq[:len(req) * 32], vox_of[:len(req) * 32], dict(
             {f: getattr(cnt, f) for f, _ in cnt._fields_}, seconds=dt)

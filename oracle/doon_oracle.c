@@ -33,6 +33,7 @@
  * by specular rays are merged after the lit chunks' bits are cleared.
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -531,13 +532,14 @@ static v3 voxel_color(const ORscene* s, uint32_t viewMode, ORvoxel vox, v3 ca, r
     }
 }
 
-void or_draw(const ORscene* s, const ORdrawParams* dp, float* rgba, ORpixel* px, uint32_t* flagsOut, ORcounters* cnt)
+static void draw_rows(const ORscene* s, const ORdrawParams* dp, uint32_t y0, uint32_t y1, float* rgba, ORpixel* px,
+                      uint32_t* flagsOut, ORcounters* cnt)
 {
     ORthread t;
     thread_init(&t, 1);
     const float* M = dp->invViewProj;
     v3 ms = V((real)s->mapSize[0], (real)s->mapSize[1], (real)s->mapSize[2]);
-    for(uint32_t y = 0; y < dp->height; y++)
+    for(uint32_t y = y0; y < y1; y++)
     for(uint32_t x = 0; x < dp->width; x++)
     {
         t.lastVoxID = 255u; t.lastVoxRefract = RL(1.0);
@@ -594,6 +596,80 @@ void or_draw(const ORscene* s, const ORdrawParams* dp, float* rgba, ORpixel* px,
         if(px) px[o] = info;
     }
     if(cnt) *cnt = t.cnt;
+}
+
+
+typedef struct DRange {
+    const ORscene* s;
+    const ORdrawParams* dp;
+    uint32_t y0, y1;
+    float* rgba;
+    ORpixel* px;
+    uint32_t* visible;      /* this range's visible-bit writes (a copy of flagsOut) */
+    uint32_t* requests;     /* this range's request writes (a copy of the scene's flagsOut) */
+    ORcounters cnt;
+} DRange;
+
+static void* draw_rows_thread(void* arg)
+{
+    DRange* j = (DRange*)arg;
+    ORscene sc = *j->s;
+    sc.flagsOut = j->requests;
+    sc.lastUsedOut = NULL;
+    draw_rows(&sc, j->dp, j->y0, j->y1, j->rgba, j->px, j->visible, &j->cnt);
+    return NULL;
+}
+
+/* The draw cast split into `numThreads` bands of rows run on host threads.
+ * Pixels are independent; the visible bits (flagsOut |= 4) and ray requests
+ * (the scene's flagsOut = 3) of the bands are merged afterwards, so the result
+ * equals or_draw's. lastUsed writes are not collected. */
+void or_draw_mt(const ORscene* s, const ORdrawParams* dp, float* rgba, ORpixel* px, uint32_t* flagsOut, ORcounters* cnt,
+                uint32_t numThreads)
+{
+    uint32_t tiles = s->mapSize[0] * s->mapSize[1] * s->mapSize[2];
+    size_t tw = tiles ? tiles : 1;
+    if(numThreads < 1) numThreads = 1;
+    if(numThreads > dp->height) numThreads = dp->height ? dp->height : 1;
+    DRange* jobs = (DRange*)calloc(numThreads, sizeof(DRange));
+    pthread_t* th = (pthread_t*)calloc(numThreads, sizeof(pthread_t));
+    for(uint32_t k = 0; k < numThreads; k++)
+    {
+        DRange* j = &jobs[k];
+        j->s = s; j->dp = dp; j->rgba = rgba; j->px = px;
+        j->y0 = (uint32_t)((uint64_t)dp->height * k / numThreads);
+        j->y1 = (uint32_t)((uint64_t)dp->height * (k + 1) / numThreads);
+        if(flagsOut)
+        {
+            j->visible = (uint32_t*)malloc(sizeof(uint32_t) * tw);
+            memcpy(j->visible, flagsOut, sizeof(uint32_t) * tiles);
+        }
+        if(s->flagsOut)
+        {
+            j->requests = (uint32_t*)malloc(sizeof(uint32_t) * tw);
+            memcpy(j->requests, s->flagsOut, sizeof(uint32_t) * tiles);
+        }
+        pthread_create(&th[k], NULL, draw_rows_thread, j);
+    }
+    for(uint32_t k = 0; k < numThreads; k++) pthread_join(th[k], NULL);
+    ORcounters total = {0, 0, 0, 0};
+    for(uint32_t k = 0; k < numThreads; k++)
+    {
+        DRange* j = &jobs[k];
+        total.calls += j->cnt.calls; total.sMap += j->cnt.sMap; total.sVox += j->cnt.sVox; total.hits += j->cnt.hits;
+        for(uint32_t i = 0; flagsOut && i < tiles; i++) flagsOut[i] |= j->visible[i] & 4u;
+        for(uint32_t i = 0; s->flagsOut && i < tiles; i++)
+            if(j->requests[i] != s->flagsOut[i] && j->requests[i] == 3u) s->flagsOut[i] = 3u;
+        free(j->visible); free(j->requests);
+    }
+    free(th);
+    free(jobs);
+    if(cnt) *cnt = total;
+}
+
+void or_draw(const ORscene* s, const ORdrawParams* dp, float* rgba, ORpixel* px, uint32_t* flagsOut, ORcounters* cnt)
+{
+    draw_rows(s, dp, 0, dp->height, rgba, px, flagsOut, cnt);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -809,36 +885,49 @@ static inline uint32_t rne_u(real x) { return (uint32_t)r_rint(x); }
 static real* g_freshOut = NULL;
 void or_set_fresh_output(real* out) { g_freshOut = out; }
 
-void or_lighting(const ORscene* s, const ORlightParams* lp, const uint32_t* requests, uint32_t numRequests,
-                 uint32_t* voxelsOut, uint32_t* mapOut, uint32_t* chunksOut, real* preq, uint32_t* voxOf,
-                 ORcounters* cnt)
+/* One contiguous range of requests of a lighting dispatch. Reads only the
+ * dispatch-start state; writes voxel words, pre-quantisation values and pool
+ * indices of its own slots, and its own per-tile arrays (lit, visible
+ * propagation, request and lastUsed writes), so ranges are independent. */
+typedef struct LRange {
+    const ORscene* s;
+    const ORlightParams* lp;
+    const uint32_t* requests;
+    uint32_t r0, r1;
+    uint32_t* voxelsOut;
+    real* preq;
+    uint32_t* voxOf;
+    uint8_t* prop;
+    uint8_t* lit;
+    uint32_t* flagsW;
+    uint32_t* lastW;
+    ORcounters cnt;
+} LRange;
+
+static void light_range(LRange* j)
 {
-    uint32_t tiles = s->mapSize[0] * s->mapSize[1] * s->mapSize[2];
-    uint8_t* prop = (uint8_t*)calloc(tiles ? tiles : 1, 1);
-    uint8_t* lit = (uint8_t*)calloc(tiles ? tiles : 1, 1);
-    uint32_t* flagsW = (uint32_t*)malloc(sizeof(uint32_t) * (tiles ? tiles : 1));
-    uint32_t* lastW = (uint32_t*)malloc(sizeof(uint32_t) * (tiles ? tiles : 1));
-    for(uint32_t i = 0; i < tiles; i++) { flagsW[i] = s->map[3 * i]; lastW[i] = s->map[3 * i + 1]; }
+    const ORscene* s = j->s;
+    const ORlightParams* lp = j->lp;
     ORscene sc = *s;
-    sc.flagsOut = flagsW;
-    sc.lastUsedOut = lastW;
+    sc.flagsOut = j->flagsW;
+    sc.lastUsedOut = j->lastW;
     ORthread t;
     thread_init(&t, 0);
     LCtx c;
-    c.s = &sc; c.lp = lp; c.t = &t; c.visProp = prop;
+    c.s = &sc; c.lp = lp; c.t = &t; c.visProp = j->prop;
     c.sunDir = vf(lp->sunDir);
     c.sunStrength = vf(s->sunStrength);
     v3 amb = vf(lp->ambient);
-    for(uint32_t r = 0; r < numRequests; r++)
+    for(uint32_t r = j->r0; r < j->r1; r++)
     {
-        uint32_t mapIndex = requests[r] >> 4;
-        uint32_t block = requests[r] & 15u;
+        uint32_t mapIndex = j->requests[r] >> 4;
+        uint32_t block = j->requests[r] & 15u;
         const uint32_t* chunk = s->chunks + (size_t)mapIndex * CHUNK_WORDS;
-        lit[mapIndex] = 1;
+        j->lit[mapIndex] = 1;
         for(uint32_t lane = 0; lane < 32; lane++)
         {
             size_t slot = (size_t)r * 32 + lane;
-            if(voxOf) voxOf[slot] = 0xFFFFFFFFu;
+            if(j->voxOf) j->voxOf[slot] = 0xFFFFFFFFu;
             uint32_t voxNum = lane + block * 32;
             int32_t sel = or_select(chunk, voxNum);
             if(sel < 0) continue;
@@ -881,12 +970,13 @@ void or_lighting(const ORscene* s, const ORlightParams* lp, const uint32_t* requ
                 }
                 diff = vdivs(vadd(vmuls(me.diffuse, indirect), diff), indirect + (real)lp->numDiffuseSamples);
             }
-            if(preq)
+            if(j->preq)
             {
-                preq[slot * 6 + 0] = diff.x; preq[slot * 6 + 1] = diff.y; preq[slot * 6 + 2] = diff.z;
-                preq[slot * 6 + 3] = spec.x; preq[slot * 6 + 4] = spec.y; preq[slot * 6 + 5] = spec.z;
+                real* q = j->preq + slot * 6;
+                q[0] = diff.x; q[1] = diff.y; q[2] = diff.z;
+                q[3] = spec.x; q[4] = spec.y; q[5] = spec.z;
             }
-            if(voxOf) voxOf[slot] = vi;
+            if(j->voxOf) j->voxOf[slot] = vi;
             spec = vclamp(spec, RL(0.0), RL(1.0));
             diff = vclamp(diff, RL(0.0), RL(1.0));
             uint32_t wdx = rne_u(diff.x * RL(65535.0)), wdy = rne_u(diff.y * RL(65535.0)), wdz = rne_u(diff.z * RL(65535.0));
@@ -894,24 +984,95 @@ void or_lighting(const ORscene* s, const ORlightParams* lp, const uint32_t* requ
                      ab = rne_u(me.albedo.z * RL(255.0)) & 255u;
             uint32_t sx = rne_u(spec.x * RL(255.0)) & 255u, sy = rne_u(spec.y * RL(255.0)) & 255u,
                      sz = rne_u(spec.z * RL(255.0)) & 255u;
-            uint32_t* ow = voxelsOut + (size_t)vi * VOX_WORDS;
+            uint32_t* ow = j->voxelsOut + (size_t)vi * VOX_WORDS;
             ow[1] = (ar << 24) | (ag << 16) | (ab << 8) | sx;
             ow[2] = (sy << 24) | (sz << 16) | (((wdx >> 8) & 255u) << 8) | (wdx & 255u);
             ow[3] = (((wdy >> 8) & 255u) << 24) | ((wdy & 255u) << 16) | (((wdz >> 8) & 255u) << 8) | (wdz & 255u);
         }
     }
+    j->cnt = t.cnt;
+}
+
+static void* light_range_thread(void* arg)
+{
+    light_range((LRange*)arg);
+    return NULL;
+}
+
+/* The lighting dispatch (voxelLighting.comp:207-288) over `numRequests`
+ * requests, split into `numThreads` contiguous ranges run on host threads.
+ * Every range reads the dispatch-start state only, so the result does not
+ * depend on the split: the per-tile writes are merged afterwards (a lit tile
+ * loses its visible bit, a tile reached by specular rays gains it, a request
+ * write sets flags 3, a loaded tile entered by a ray gets lastUsed 0). */
+void or_lighting_mt(const ORscene* s, const ORlightParams* lp, const uint32_t* requests, uint32_t numRequests,
+                    uint32_t* voxelsOut, uint32_t* mapOut, uint32_t* chunksOut, real* preq, uint32_t* voxOf,
+                    ORcounters* cnt, uint32_t numThreads)
+{
+    uint32_t tiles = s->mapSize[0] * s->mapSize[1] * s->mapSize[2];
+    size_t tw = tiles ? tiles : 1;
+    if(numThreads < 1) numThreads = 1;
+    if(numThreads > numRequests) numThreads = numRequests ? numRequests : 1;
+    LRange* jobs = (LRange*)calloc(numThreads, sizeof(LRange));
+    for(uint32_t k = 0; k < numThreads; k++)
+    {
+        LRange* j = &jobs[k];
+        j->s = s; j->lp = lp; j->requests = requests;
+        j->r0 = (uint32_t)((uint64_t)numRequests * k / numThreads);
+        j->r1 = (uint32_t)((uint64_t)numRequests * (k + 1) / numThreads);
+        j->voxelsOut = voxelsOut; j->preq = preq; j->voxOf = voxOf;
+        j->prop = (uint8_t*)calloc(tw, 1);
+        j->lit = (uint8_t*)calloc(tw, 1);
+        j->flagsW = (uint32_t*)malloc(sizeof(uint32_t) * tw);
+        j->lastW = (uint32_t*)malloc(sizeof(uint32_t) * tw);
+        for(uint32_t i = 0; i < tiles; i++) { j->flagsW[i] = s->map[3 * i]; j->lastW[i] = s->map[3 * i + 1]; }
+    }
+    if(numThreads == 1) light_range(&jobs[0]);
+    else
+    {
+        pthread_t* th = (pthread_t*)calloc(numThreads, sizeof(pthread_t));
+        for(uint32_t k = 0; k < numThreads; k++) pthread_create(&th[k], NULL, light_range_thread, &jobs[k]);
+        for(uint32_t k = 0; k < numThreads; k++) pthread_join(th[k], NULL);
+        free(th);
+    }
+    ORcounters total = {0, 0, 0, 0};
+    for(uint32_t k = 0; k < numThreads; k++)
+    {
+        total.calls += jobs[k].cnt.calls; total.sMap += jobs[k].cnt.sMap;
+        total.sVox += jobs[k].cnt.sVox; total.hits += jobs[k].cnt.hits;
+    }
     for(uint32_t i = 0; i < tiles; i++)
     {
-        uint32_t f = flagsW[i];
-        if(lit[i]) f &= ~4u;
-        if(prop[i]) f |= 4u;
+        uint32_t f = s->map[3 * i], last = s->map[3 * i + 1];
+        int lit = 0, prop = 0;
+        for(uint32_t k = 0; k < numThreads; k++)
+        {
+            const LRange* j = &jobs[k];
+            if(j->flagsW[i] != s->map[3 * i]) f = j->flagsW[i];
+            if(j->lastW[i] < last) last = j->lastW[i];
+            lit |= j->lit[i];
+            prop |= j->prop[i];
+        }
+        if(lit) f &= ~4u;
+        if(prop) f |= 4u;
         mapOut[3 * i] = f;
-        mapOut[3 * i + 1] = lastW[i];
+        mapOut[3 * i + 1] = last;
         mapOut[3 * i + 2] = s->map[3 * i + 2];
-        if(lit[i]) chunksOut[(size_t)i * CHUNK_WORDS + 3] = s->chunks[(size_t)i * CHUNK_WORDS + 3] + 1u;
+        if(lit) chunksOut[(size_t)i * CHUNK_WORDS + 3] = s->chunks[(size_t)i * CHUNK_WORDS + 3] + 1u;
     }
-    if(cnt) *cnt = t.cnt;
-    free(prop); free(lit); free(flagsW); free(lastW);
+    for(uint32_t k = 0; k < numThreads; k++)
+    {
+        free(jobs[k].prop); free(jobs[k].lit); free(jobs[k].flagsW); free(jobs[k].lastW);
+    }
+    free(jobs);
+    if(cnt) *cnt = total;
+}
+
+void or_lighting(const ORscene* s, const ORlightParams* lp, const uint32_t* requests, uint32_t numRequests,
+                 uint32_t* voxelsOut, uint32_t* mapOut, uint32_t* chunksOut, real* preq, uint32_t* voxOf,
+                 ORcounters* cnt)
+{
+    or_lighting_mt(s, lp, requests, numRequests, voxelsOut, mapOut, chunksOut, preq, voxOf, cnt, 1);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1070,6 +1231,18 @@ void or_set_voxel_word(ORgrid* g, int x, int y, int z, uint32_t normal, uint32_t
     g->albedo[v] = albedo;
 }
 
+/* DNX_set_voxels_dense: or_set_voxel_word over a box of sx*sy*sz voxels from
+ * (ox, oy, oz), x fastest, then y, then z (the words in that order). */
+void or_set_dense(ORgrid* g, int ox, int oy, int oz, int sx, int sy, int sz, const uint32_t* normal,
+                  const uint32_t* albedo)
+{
+    size_t k = 0;
+    for(int z = 0; z < sz; z++)
+    for(int y = 0; y < sy; y++)
+    for(int x = 0; x < sx; x++, k++)
+        or_set_voxel_word(g, ox + x, oy + y, oz + z, normal[k], albedo[k]);
+}
+
 /* DN_compress_voxel, voxel.c:1289-1301 */
 static uint32_t g_compress_normal(uint32_t material, float nx, float ny, float nz)
 {
@@ -1080,7 +1253,63 @@ static uint32_t g_compress_normal(uint32_t material, float nx, float ny, float n
     return (material << 24) | (qx << 16) | (qy << 8) | qz;
 }
 
-/* DN_shape_sphere without the transform callback, voxelShapes.c:239-333: every
+/* The transform callback's view of a voxel and vectors: DNvoxel / DNvec3 /
+ * DNmat4 of voxel.h and quickmath.h, by value (voxelShapes.h:28). */
+typedef struct { float x, y, z; } ORv3f;
+typedef struct { uint8_t material; float normal[3]; uint8_t albedo[3]; } ORvoxelT;
+typedef struct { float m[4][4]; } ORmat4;
+typedef ORvoxelT (*ORtransformFn)(ORv3f pos, ORv3f normal, ORvoxelT vox, ORv3f shapeMin, ORv3f shapeMax,
+                                  ORmat4 invTransform, void* userData);
+
+/* voxelShapes.c:242-246: inverse of translate(c) * quaternion_to_mat4(orient),
+ * QuickMath's translate (quickmath.h:1138-1147), quaternion conversion
+ * (:1622-1650), column-major product (:884-917) and cofactor inverse
+ * (:1037-1124), column-major m[col][row]. */
+static void sphere_inv_transform(float cx, float cy, float cz, const float* q, float out[4][4])
+{
+    float R[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+    if(q)
+    {
+        float x2 = q[0] + q[0], y2 = q[1] + q[1], z2 = q[2] + q[2];
+        float xx2 = q[0] * x2, xy2 = q[0] * y2, xz2 = q[0] * z2, yy2 = q[1] * y2, yz2 = q[1] * z2, zz2 = q[2] * z2;
+        float sx2 = q[3] * x2, sy2 = q[3] * y2, sz2 = q[3] * z2;
+        R[0][0] = 1.0f - (yy2 + zz2); R[0][1] = xy2 - sz2; R[0][2] = xz2 + sy2;
+        R[1][0] = xy2 + sz2; R[1][1] = 1.0f - (xx2 + zz2); R[1][2] = yz2 - sx2;
+        R[2][0] = xz2 - sy2; R[2][1] = yz2 + sx2; R[2][2] = 1.0f - (xx2 + yy2);
+    }
+    float T[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {cx, cy, cz, 1}};
+    float M[4][4];
+    for(int c = 0; c < 4; c++)
+        for(int r = 0; r < 4; r++)
+            M[c][r] = T[0][r] * R[c][0] + T[1][r] * R[c][1] + T[2][r] * R[c][2] + T[3][r] * R[c][3];
+    float a = M[0][0], b = M[0][1], c = M[0][2], d = M[0][3], e = M[1][0], f = M[1][1], g = M[1][2], h = M[1][3],
+          i = M[2][0], j = M[2][1], k = M[2][2], l = M[2][3], m = M[3][0], n = M[3][1], o = M[3][2], p = M[3][3];
+    float t0 = k * p - o * l, t1 = j * p - n * l, t2 = j * o - n * k, t3 = i * p - m * l, t4 = i * o - m * k,
+          t5 = i * n - m * j;
+    out[0][0] = f * t0 - g * t1 + h * t2;
+    out[1][0] = -(e * t0 - g * t3 + h * t4);
+    out[2][0] = e * t1 - f * t3 + h * t5;
+    out[3][0] = -(e * t2 - f * t4 + g * t5);
+    out[0][1] = -(b * t0 - c * t1 + d * t2);
+    out[1][1] = a * t0 - c * t3 + d * t4;
+    out[2][1] = -(a * t1 - b * t3 + d * t5);
+    out[3][1] = a * t2 - b * t4 + c * t5;
+    t0 = g * p - o * h; t1 = f * p - n * h; t2 = f * o - n * g; t3 = e * p - m * h; t4 = e * o - m * g; t5 = e * n - m * f;
+    out[0][2] = b * t0 - c * t1 + d * t2;
+    out[1][2] = -(a * t0 - c * t3 + d * t4);
+    out[2][2] = a * t1 - b * t3 + d * t5;
+    out[3][2] = -(a * t2 - b * t4 + c * t5);
+    t0 = g * l - k * h; t1 = f * l - j * h; t2 = f * k - j * g; t3 = e * l - i * h; t4 = e * k - i * g; t5 = e * j - i * f;
+    out[0][3] = -(b * t0 - c * t1 + d * t2);
+    out[1][3] = a * t0 - c * t3 + d * t4;
+    out[2][3] = -(a * t1 - b * t3 + d * t5);
+    out[3][3] = a * t2 - b * t4 + c * t5;
+    float det = 1.0f / (a * out[0][0] + b * out[1][0] + c * out[2][0] + d * out[3][0]);
+    for(int x = 0; x < 4; x++)
+        for(int y = 0; y < 4; y++) out[x][y] = out[x][y] * det;
+}
+
+/* DN_shape_sphere, voxelShapes.c:239-333: every
  * voxel of the (clamped) tile box, tile by tile in z, y, x order and voxel by
  * voxel in z, y, x order inside a tile; FP32 squared distance from the centre,
  * dot = (x*x + y*y) + z*z. Solid spheres skip voxels that exist and write the
@@ -1088,9 +1317,12 @@ static uint32_t g_compress_normal(uint32_t material, float nx, float ny, float n
  * voxel over every voxel inside and,
  * with flipNormals, turn the normals of the voxels in the next shell inwards.
  * The chunk-present flag is read once per tile, as the reference does. */
-void or_shape_sphere(ORgrid* g, uint32_t material, const float* normal, uint32_t albedoWord, int flipNormals,
-                     float cx, float cy, float cz, float r)
+void or_shape_sphere_fn(ORgrid* g, uint32_t material, const float* normal, uint32_t albedoWord, int flipNormals,
+                        float cx, float cy, float cz, float r, const float* orient, ORtransformFn fn, void* userData)
 {
+    ORmat4 inv;
+    sphere_inv_transform(cx, cy, cz, orient, inv.m);
+    const ORv3f shapeMin = {cx - r, cy - r, cz - r}, shapeMax = {cx + r, cy + r, cz + r};
     int iMin[3] = {(int)floorf(cx - r), (int)floorf(cy - r), (int)floorf(cz - r)};
     int iMax[3] = {(int)ceilf(cx + r), (int)ceilf(cy + r), (int)ceilf(cz + r)};
     if(material == 255u)
@@ -1122,23 +1354,56 @@ void or_shape_sphere(ORgrid* g, uint32_t material, const float* normal, uint32_t
             {
                 if(tileFlag && (g->normal[v] >> 24) != 255u && material != 255u) continue;
                 float nx = normal[0], ny = normal[1], nz = normal[2];
+                uint32_t mt = material, aw = albedoWord;
                 if(material != 255u)
                 {
                     float m = fmaxf(fmaxf(fabsf(fx), fabsf(fy)), fabsf(fz));
                     float s = 1.0f / m;
                     nx = fx * s; ny = fy * s; nz = fz * s;
+                    if(fn)
+                    {
+                        ORvoxelT in = {(uint8_t)material, {normal[0], normal[1], normal[2]},
+                                       {(uint8_t)(albedoWord >> 24), (uint8_t)(albedoWord >> 16), (uint8_t)(albedoWord >> 8)}};
+                        ORv3f pos = {(float)x, (float)y, (float)z}, n = {nx, ny, nz};
+                        ORvoxelT o = fn(pos, n, in, shapeMin, shapeMax, inv, userData);
+                        mt = o.material; nx = o.normal[0]; ny = o.normal[1]; nz = o.normal[2];
+                        aw = ((uint32_t)o.albedo[0] << 24) | ((uint32_t)o.albedo[1] << 16) | ((uint32_t)o.albedo[2] << 8);
+                    }
                 }
-                or_set_voxel_word(g, x, y, z, g_compress_normal(material, nx, ny, nz), albedoWord);
+                or_set_voxel_word(g, x, y, z, g_compress_normal(mt, nx, ny, nz), aw);
             }
             else if(material == 255u && flipNormals && d2 < r12)
             {
                 if(!tileFlag || (g->normal[v] >> 24) == 255u) continue;
                 float m = fmaxf(fmaxf(fabsf(fx), fabsf(fy)), fabsf(fz));
                 float s = -1.0f / m;
+                if(fn)
+                {
+                    /* DN_get_voxel decodes the old voxel (voxel.c:1304-1316), func gets it */
+                    const float inv255 = 1.0f / 255.0f;
+                    uint32_t w = g->normal[v], al = g->albedo[v];
+                    ORvoxelT old = {(uint8_t)(w >> 24),
+                                    {(float)((int)((w >> 16) & 255u) * 2 - 255) * inv255,
+                                     (float)((int)((w >> 8) & 255u) * 2 - 255) * inv255,
+                                     (float)((int)(w & 255u) * 2 - 255) * inv255},
+                                    {(uint8_t)(al >> 24), (uint8_t)(al >> 16), (uint8_t)(al >> 8)}};
+                    ORv3f pos = {(float)x, (float)y, (float)z}, n = {fx * s, fy * s, fz * s};
+                    ORvoxelT o = fn(pos, n, old, shapeMin, shapeMax, inv, userData);
+                    or_set_voxel_word(g, x, y, z, g_compress_normal(o.material, o.normal[0], o.normal[1], o.normal[2]),
+                                      ((uint32_t)o.albedo[0] << 24) | ((uint32_t)o.albedo[1] << 16) |
+                                          ((uint32_t)o.albedo[2] << 8));
+                    continue;
+                }
                 /* DN_get_voxel then DN_set_voxel: the albedo keeps its three colour bytes */
                 or_set_voxel_word(g, x, y, z, g_compress_normal(g->normal[v] >> 24, fx * s, fy * s, fz * s),
                                   g->albedo[v] & 0xFFFFFF00u);
             }
         }
     }
+}
+
+void or_shape_sphere(ORgrid* g, uint32_t material, const float* normal, uint32_t albedoWord, int flipNormals,
+                     float cx, float cy, float cz, float r)
+{
+    or_shape_sphere_fn(g, material, normal, albedoWord, flipNormals, cx, cy, cz, r, NULL, NULL, NULL);
 }

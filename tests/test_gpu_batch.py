@@ -114,3 +114,38 @@ def test_autotune_leaves_state(gpu):
         batch.close()
         for v in va + vb:
             v.close()
+
+
+def test_autotune_leaves_unloaded_chunks_alone(gpu):
+    """Autotuning runs compute-only passes: with chunks present on the CPU but
+    not loaded (a roof added after the first sync), whose tiles the lighting
+    rays enter, the dry passes must neither request them (flags 3)
+    nor stamp lastUsed; the real pass that follows does request them."""
+    vols, vis = build_set()
+    try:
+        for v in vols:
+            # a one-voxel roof over the whole map: the shadow rays that leave the
+            # terrain upwards cross its (new, unloaded) chunks
+            X, Y, Z = (8 * c for c in v.map_size)
+            nw = np.full((Z, 1, X), (1 << 24) | (127 << 16) | (255 << 8) | 127, np.uint32)
+            aw = np.full((Z, 1, X), 0xC8C8C800, np.uint32)
+            v.set_dense(nw, aw, origin=(0, Y - 1, 0))
+        frame_sync(vols, vis)
+        for v in vols:
+            v.synchronize()
+        before = [v.export_state() for v in vols]
+        assert any(((m[:, 0] & 3) == 1).any() for m, _, _ in before)       # unloaded chunks exist
+        batch = Batch(vols)
+        try:
+            batch.autotune(1, 1000, 321.0, repeats=1)
+            for v, (m0, c0, p0) in zip(vols, before):
+                m1, c1, p1 = v.export_state()
+                assert np.array_equal(m1, m0) and np.array_equal(c1, c0) and np.array_equal(p1, p0)
+            batch.update_lighting(1, 1000, 321.0)
+            requested = sum(int(((v.export_state()[0][:, 0] & 3) == 3).sum()) for v in vols)
+            assert requested > 0
+        finally:
+            batch.close()
+    finally:
+        for v in vols:
+            v.close()

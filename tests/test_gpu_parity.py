@@ -133,24 +133,47 @@ def test_rays_parity(gpu, small_scene):
     vol.close()
 
 
-def run_frame_and_compare(vol, sc, o, frame, time, w, h, num_diffuse=1, max_diffuse=1000, split=1):
-    """draw -> sync(READ) -> lighting on the GPU, each pass checked against the oracle."""
+def run_frame_and_compare(vol, sc, o, frame, time, w, h, num_diffuse=1, max_diffuse=1000, split=1, threads=1,
+                          mark_visible=0, full_draw=False, states=None):
+    """draw -> sync(READ_WRITE) -> lighting on the GPU, each pass checked against
+    the oracle (`threads` host threads for the oracle's passes). The draw's and
+    the lighting rays' requests of unloaded chunks (flags = 3) are part of the
+    compared flags. mark_visible > 0 also marks up to that many exposed chunks
+    visible after the draw, as the full-frame workload does. full_draw compares
+    the whole draw record (face, trip counts, depth, colour), not only the hit
+    voxels. `states`, a dict, receives the exported state after the draw (m1,
+    c1) and after the sync (m2, c2)."""
     s, (m0, c0, v0) = oracle_scene(o, vol, sc)
     rgba, px, view, proj = draw_gpu(vol, sc, w, h)
     from doonengine_amd.volume import draw_matrices
     iv, ivp = draw_matrices(view, proj)
     flags = m0[:, 0].copy()
-    orgba, opx, _ = o.draw(s, mat_to_np(iv), mat_to_np(ivp), w, h, flags_out=flags)
+    s.flagsOut = flags.ctypes.data                         # ray requests (flags = 3) land in the same words
+    orgba, opx, dcnt = o.draw(s, mat_to_np(iv), mat_to_np(ivp), w, h, flags_out=flags, threads=threads)
+    s.flagsOut = None
     assert np.array_equal(px["voxelIndex"], opx["voxelIndex"])
+    if full_draw:
+        assert np.array_equal(px["mapSteps"], opx["mapSteps"])
+        assert np.array_equal(px["voxelSteps"], opx["voxelSteps"])
+        assert np.array_equal(px["face"].view(np.uint32) & 0x7FFFFFFF, opx["face"].view(np.uint32) & 0x7FFFFFFF)
+        assert np.array_equal(rgba[..., 3], orgba[..., 3])
+        np.testing.assert_allclose(rgba[..., :3], orgba[..., :3], rtol=2e-6, atol=1e-7)
+        dc = vol.counters(0, reset=True)
+        assert dc["rays"] == dcnt["calls"] and dc["mapSteps"] == dcnt["sMap"] and dc["voxelSteps"] == dcnt["sVox"]
     # visible-chunk pass
     m1, c1, v1 = vol.export_state()
     assert np.array_equal(m1[:, 0], flags)
+    if mark_visible:
+        vol.mark_visible(mark_visible)
+        m1, c1, v1 = vol.export_state()
     vol.sync(op=2, split=split)
     req = vol.fetch_requests()
     ref_req = o.visible_requests(m1, c1, split=split, frame=int(vol.v.frameNum))
     assert np.array_equal(req, ref_req)
     m2, c2, v2 = vol.export_state()
     assert np.array_equal(m2[:, 1], m1[:, 1] + 1)          # lastUsed++ on every tile
+    if states is not None:
+        states.update(m1=m1, c1=c1, m2=m2, c2=c2)
     # lighting pass
     R = max(len(req), 1)
     dbg = DeviceBuffer(R * 32 * 6 * 4)
@@ -169,7 +192,8 @@ def run_frame_and_compare(vol, sc, o, frame, time, w, h, num_diffuse=1, max_diff
         diffuse_bounces=int(vol.v.diffuseBounceLimit), spec_bounces=int(vol.v.specBounceLimit),
         sun_dir=sun_dir_normalized(vol),
         ambient=tuple(getattr(vol.v.ambientLightStrength, k) for k in "xyz"), softness=float(vol.v.shadowSoftness),
-        cam_pos=tuple(getattr(vol.v.camPos, k) for k in "xyz"), spec_points=spec_pts, direct_points=direct)
+        cam_pos=tuple(getattr(vol.v.camPos, k) for k in "xyz"), spec_points=spec_pts, direct_points=direct,
+        threads=threads)
     m3, c3, v3 = vol.export_state()
     assert np.array_equal(g_idx, voxof)
     valid = voxof != 0xFFFFFFFF

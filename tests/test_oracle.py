@@ -569,3 +569,44 @@ def test_sphere_points_table_is_near_golden_angle():
     assert pts.shape == (15, 3)
     np.testing.assert_allclose(pts, oracle.golden_points(15), atol=1.5e-6)
     np.testing.assert_allclose(np.linalg.norm(pts, axis=1), 1.0, atol=3e-6)
+
+
+# ---------------------------------------------------------------------------
+# host-thread splits of the oracle's draw and lighting (used for the full-size
+# parity checks) give the single-thread result exactly
+
+def test_threaded_oracle_equals_single_thread(o, o64):
+    sc = scenes.config_small(seed=12)
+    sc.materials[1].specular = 0.7                    # specular rays: visible-bit propagation across ranges
+    sc.materials[1].shininess = 3
+    m, c, v, mats = build_state(sc)
+    for k in range(4):
+        mats[k]["specular"] = sc.materials[k].specular
+        mats[k]["shininess"] = sc.materials[k].shininess
+    _, _, iv, ivp = camera(sc)
+    for orc in (o, o64):
+        s = orc.scene(sc.map_size, m, c, v, mats)
+        f1 = m[:, 0].copy()
+        r1, p1, c1 = orc.draw(s, iv, ivp, sc.width, sc.height, flags_out=f1)
+        for th in (3, 7):
+            fk = m[:, 0].copy()
+            rk, pk, ck = orc.draw(s, iv, ivp, sc.width, sc.height, flags_out=fk, threads=th)
+            assert np.array_equal(r1.view(np.uint32), rk.view(np.uint32)) and np.array_equal(p1, pk)
+            assert np.array_equal(f1, fk)
+            assert {k: ck[k] for k in ("calls", "sMap", "sVox", "hits")} == \
+                {k: c1[k] for k in ("calls", "sMap", "sVox", "hits")}
+        mm = m.copy()
+        mm[:, 0] = f1
+        req = orc.visible_requests(mm, c)
+        assert len(req) > 20
+        s2 = orc.scene(sc.map_size, mm, c, v, mats)
+        kw = dict(time=321.0, sun_dir=np.array([-0.57735, 0.57735, -0.57735], np.float32),
+                  spec_points=oracle.golden_points(15).astype(np.float32), cam_pos=sc.cam_pos)
+        base = orc.lighting(s2, req, **kw)
+        assert (base[1][:, 0] != mm[:, 0]).any()          # some visible bits change
+        for th in (2, 5, len(req) + 3):
+            got = orc.lighting(s2, req, threads=th, **kw)
+            for a, b in zip(base[:5], got[:5]):
+                assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
+            assert {k: got[5][k] for k in ("calls", "sMap", "sVox", "hits")} == \
+                {k: base[5][k] for k in ("calls", "sMap", "sVox", "hits")}
