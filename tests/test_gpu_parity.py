@@ -133,6 +133,20 @@ def test_rays_parity(gpu, small_scene):
     vol.close()
 
 
+def oracle_lighting(o, vol, s, req, num_diffuse=1, max_diffuse=1000, threads=1):
+    """The oracle's lighting dispatch on scene `s` with the volume's own lighting
+    parameters (the time it latched, bounce limits, sun, ambient, softness,
+    camera, specular and direct ray tables)."""
+    direct = vol.ray_points(1) if vol.v.numDirectRays else None
+    return o.lighting(
+        s, req, time=float(vol.v.lastTime), num_diffuse=num_diffuse, max_diffuse=max_diffuse,
+        diffuse_bounces=int(vol.v.diffuseBounceLimit), spec_bounces=int(vol.v.specBounceLimit),
+        sun_dir=sun_dir_normalized(vol),
+        ambient=tuple(getattr(vol.v.ambientLightStrength, k) for k in "xyz"), softness=float(vol.v.shadowSoftness),
+        cam_pos=tuple(getattr(vol.v.camPos, k) for k in "xyz"), spec_points=vol.ray_points(0), direct_points=direct,
+        threads=threads)
+
+
 def run_frame_and_compare(vol, sc, o, frame, time, w, h, num_diffuse=1, max_diffuse=1000, split=1, threads=1,
                           mark_visible=0, full_draw=False, states=None):
     """draw -> sync(READ_WRITE) -> lighting on the GPU, each pass checked against
@@ -143,6 +157,8 @@ def run_frame_and_compare(vol, sc, o, frame, time, w, h, num_diffuse=1, max_diff
     the whole draw record (face, trip counts, depth, colour), not only the hit
     voxels. `states`, a dict, receives the exported state after the draw (m1,
     c1) and after the sync (m2, c2)."""
+    vol.counters(0, reset=True)                            # counts of earlier, unchecked passes
+    vol.counters(1, reset=True)
     s, (m0, c0, v0) = oracle_scene(o, vol, sc)
     rgba, px, view, proj = draw_gpu(vol, sc, w, h)
     from doonengine_amd.volume import draw_matrices
@@ -171,7 +187,13 @@ def run_frame_and_compare(vol, sc, o, frame, time, w, h, num_diffuse=1, max_diff
     ref_req = o.visible_requests(m1, c1, split=split, frame=int(vol.v.frameNum))
     assert np.array_equal(req, ref_req)
     m2, c2, v2 = vol.export_state()
-    assert np.array_equal(m2[:, 1], m1[:, 1] + 1)          # lastUsed++ on every tile
+    # lastUsed++ on every tile (voxel.c:753), except chunks (re)loaded by this sync, which restart at 0
+    # (voxel.c:1526); every chunk that became loaded is one of them
+    aged = m2[:, 1] == m1[:, 1] + 1
+    restarted = ~aged
+    assert np.all(m2[restarted, 1] == 0) and np.all((m2[restarted, 0] & 3) == 2)
+    newly = ((m2[:, 0] & 3) == 2) & ((m1[:, 0] & 3) != 2)
+    assert np.all(restarted[newly])
     if states is not None:
         states.update(m1=m1, c1=c1, m2=m2, c2=c2)
     # lighting pass
@@ -185,15 +207,7 @@ def run_frame_and_compare(vol, sc, o, frame, time, w, h, num_diffuse=1, max_diff
     vol.lib.DNX_set_lighting_debug_output(vol.ptr, None, None, 0)
     g_val = dbg.download(np.float32, (R * 32, 6))[: len(req) * 32]
     g_idx = dbi.download(np.uint32, (R * 32,))[: len(req) * 32]
-    spec_pts = vol.ray_points(0)
-    direct = vol.ray_points(1) if vol.v.numDirectRays else None
-    vo, mo, co, preq, voxof, ocnt = o.lighting(
-        s2, req, time=float(vol.v.lastTime), num_diffuse=num_diffuse, max_diffuse=max_diffuse,
-        diffuse_bounces=int(vol.v.diffuseBounceLimit), spec_bounces=int(vol.v.specBounceLimit),
-        sun_dir=sun_dir_normalized(vol),
-        ambient=tuple(getattr(vol.v.ambientLightStrength, k) for k in "xyz"), softness=float(vol.v.shadowSoftness),
-        cam_pos=tuple(getattr(vol.v.camPos, k) for k in "xyz"), spec_points=spec_pts, direct_points=direct,
-        threads=threads)
+    vo, mo, co, preq, voxof, ocnt = oracle_lighting(o, vol, s2, req, num_diffuse, max_diffuse, threads)
     m3, c3, v3 = vol.export_state()
     assert np.array_equal(g_idx, voxof)
     valid = voxof != 0xFFFFFFFF

@@ -71,7 +71,7 @@ def test_bounded_pool_evicts_least_recently_used(gpu):
     sc = scenes.config_small(seed=22, chunks=(8, 3, 8), width=48, height=40)
     sc.cam_pos = (4.0, 2.6, 4.0)
     o = oracle.load("f32")
-    rm, rc, rv, _ = build_state(sc, materials_array(sc), node=16)
+    rm, rc, _, _ = build_state(sc, materials_array(sc))
     present = (rm[:, 0] & 3) == 2
     need = 0
     for t in np.nonzero(present)[0]:
@@ -82,7 +82,6 @@ def test_bounded_pool_evicts_least_recently_used(gpu):
     try:
         sc.apply(vol)
         assert vol.set_pool_limit(limit)
-        assert not vol.set_pool_limit(256)                     # below the pool's current size
         vol.sync()
         evicted_seen = deferred_seen = 0
         for f in range(16):
@@ -108,6 +107,8 @@ def test_bounded_pool_evicts_least_recently_used(gpu):
         stats = vol.residency_stats()
         assert stats["evictions"] == evicted_seen and stats["deferrals"] == deferred_seen
         assert evicted_seen > 0
+        assert vol.v.voxelCap > 512 and not vol.set_pool_limit(512)   # below the pool's current size
+        assert vol.set_pool_limit(limit)
         print(f"[stream] pool {vol.v.voxelCap} of limit {limit} (map needs {need}), {evicted_seen} evictions, "
               f"{deferred_seen} deferred loads", flush=True)
     finally:
