@@ -204,6 +204,7 @@ SIGNATURES = {
     "DNX_rc_predict_requests": (c_bool, [c_void_p, c_void_p, c_size_t]),
     "DNX_rc_train_samples": (c_bool, [c_void_p, c_void_p, c_uint32, c_bool]),
     "DNX_rc_predict_samples": (c_bool, [c_void_p, c_void_p, c_uint32, c_void_p]),
+    "DNX_rc_predict_samples_noalloc": (c_bool, [c_void_p, c_void_p, c_uint32, c_void_p]),
     "DNX_rc_set_params": (c_bool, [c_void_p, c_void_p, c_void_p]),
     "DNX_rc_get_params": (c_bool, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "DNX_rc_last_samples": (c_void_p, [c_void_p, POINTER(c_uint32)]),

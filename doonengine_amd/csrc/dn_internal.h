@@ -229,6 +229,13 @@ int dn_rc_encode(const DNrcNet& net, const float* samples, uint32_t n, uint16_t*
 int dn_rc_forward(const DNrcNet& net, const uint16_t* x0, const float* samples, uint32_t n, float* pred,
                   hipStream_t stream);
 int dn_rc_train(const DNrcNet& net, const float* samples, uint32_t n, const DNrcActs& a, hipStream_t stream);
+// Allocation-free prediction: dn_rc_encode + dn_rc_forward with the encoded rows
+// in the fixed per-device scratch array of rcache.hip (at most
+// DNX_RC_NOALLOC_MAX_SAMPLES rows) on device `device`. dn_noalloc_init (called
+// by DN_init) looks up every device's address of that array first.
+bool dn_noalloc_init();
+int dn_rc_predict_noalloc(const DNrcNet& net, int device, const float* samples, uint32_t n, float* pred,
+                          hipStream_t stream);
 int dn_rc_adam(const DNrcNet& net, const DNrcAdam& opt, hipStream_t stream);
 int dn_rc_samples(const DNlightParams& p, uint32_t maxRequests, uint32_t n, uint32_t seed, float* samples,
                   hipStream_t stream);

@@ -755,6 +755,9 @@ extern "C" bool DN_init(void)
         return false;
     }
     build_lut();
+    // every device's address of the allocation-free prediction's scratch array
+    if(!dn_noalloc_init())
+        dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR, "allocation-free prediction: scratch address lookup failed");
     return true;
 }
 

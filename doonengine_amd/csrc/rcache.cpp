@@ -14,6 +14,7 @@ void dn_msg(DNmessageType type, DNmessageSeverity sev, const char* fmt, ...);
 struct DNXradianceCache {
     DNvolume* vol = nullptr;
     hipStream_t stream = nullptr;
+    int device = 0;               // the device its buffers live on (current at creation)
     DNXrcConfig cfg;
     DNrcNet net;
     DNrcActs acts;
@@ -215,6 +216,7 @@ extern "C" DNXradianceCache* DNX_rc_create(DNvolume* vol, const DNXrcConfig* con
     std::memset(&rc->acts, 0, sizeof(rc->acts));
     rc->vol = vol;
     rc->stream = (hipStream_t)DNX_get_stream(vol);
+    if(hipGetDevice(&rc->device) != hipSuccess) rc->device = 0;
     if(c.extent <= 0.0f)
     {
         uint32_t m = vol->mapSize.x > vol->mapSize.y ? vol->mapSize.x : vol->mapSize.y;
@@ -302,6 +304,25 @@ extern "C" bool DNX_rc_predict_samples(DNXradianceCache* rc, const float* sample
     const DNrcNet inf = infer_net(rc);
     return dn_rc_encode(inf, samples, n, rc->dQueryX0, rc->stream) == 0 &&
            dn_rc_forward(inf, rc->dQueryX0, samples, n, out, rc->stream) == 0;
+}
+
+extern "C" bool DNX_rc_predict_samples_noalloc(DNXradianceCache* rc, const float* samples, uint32_t n, float* out)
+{
+    if(!rc || !samples || !out || n == 0 || (n & 15u)) return false;
+    if(n > DNX_RC_NOALLOC_MAX_SAMPLES)
+    {
+        dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR,
+               "radiance cache: %u samples exceed the allocation-free prediction's maximum of %u", n,
+               DNX_RC_NOALLOC_MAX_SAMPLES);
+        return false;
+    }
+    const int e = dn_rc_predict_noalloc(infer_net(rc), rc->device, samples, n, out, rc->stream);
+    if(e == (int)hipErrorNotInitialized)
+        dn_msg(DN_MESSAGE_GPU_MEMORY, DN_MESSAGE_ERROR,
+               "radiance cache: no scratch array on device %d (DN_init was not called)", rc->device);
+    else if(e)
+        dn_msg(DN_MESSAGE_SHADER, DN_MESSAGE_ERROR, "radiance cache: allocation-free prediction failed (%d)", e);
+    return e == 0;
 }
 
 extern "C" bool DNX_rc_train_frame(DNXradianceCache* rc, float time, int steps)

@@ -14,6 +14,10 @@
 // split-K MFMA reduction over the tiles, hash-table gradients by FP32 atomics,
 // then Adam (Kingma & Ba 2015, algorithm 1) on FP32 master copies.
 #include "dn_light.hpp"
+#include "doon/doon.h"
+
+#include <mutex>
+#include <vector>
 
 namespace {
 
@@ -745,4 +749,57 @@ int dn_accumulate_f64(const float* src, double* dst, size_t n, hipStream_t strea
     if(n == 0) return 0;
     hipLaunchKernelGGL(k_accumulate_f64, dim3(blocks_for(n, 4096)), dim3(256), 0, stream, src, dst, n);
     return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free prediction (DNX_rc_predict_samples_noalloc). The encoded
+// input rows that DNX_rc_predict_samples keeps in a query buffer grown with
+// hipMalloc live here in `scratch`, a fixed array that every device holds in
+// its own memory for the life of the process: DNX_RC_NOALLOC_MAX_SAMPLES rows
+// of RC_IN bf16 values (24 MiB). k_rc_encode writes every column of the rows
+// it is given and k_rc_mlp reads only those, so nothing needs clearing between
+// calls. In host code `scratch` names a host-side stand-in, not device memory:
+// kernels get the address that hipGetSymbolAddress gives for each device.
+static_assert(DNX_RC_NOALLOC_MAX_SAMPLES % 16u == 0u, "whole 16-sample tiles");
+static __device__ __align__(256) uint16_t scratch[(size_t)DNX_RC_NOALLOC_MAX_SAMPLES * RC_IN];
+
+namespace {
+std::mutex g_scratchLock;
+std::vector<uint16_t*> g_scratchAddr;     // per device: scratch's device address
+}  // namespace
+
+bool dn_noalloc_init()
+{
+    int count = 0, current = 0;
+    if(hipGetDeviceCount(&count) != hipSuccess || count < 1 || hipGetDevice(&current) != hipSuccess) return false;
+    std::lock_guard<std::mutex> lock(g_scratchLock);
+    if(g_scratchAddr.size() < (size_t)count) g_scratchAddr.resize((size_t)count, nullptr);
+    bool ok = true;
+    for(int d = 0; d < count; d++)
+    {
+        if(g_scratchAddr[d]) continue;                       // once per device
+        void* p = nullptr;
+        if(hipSetDevice(d) != hipSuccess || hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+        {
+            ok = false;
+            continue;
+        }
+        g_scratchAddr[d] = static_cast<uint16_t*>(p);
+    }
+    if(hipSetDevice(current) != hipSuccess) ok = false;
+    return ok;
+}
+
+int dn_rc_predict_noalloc(const DNrcNet& net, int device, const float* samples, uint32_t n, float* pred,
+                          hipStream_t stream)
+{
+    if(n == 0 || (n & 15u) || n > DNX_RC_NOALLOC_MAX_SAMPLES) return (int)hipErrorInvalidValue;
+    uint16_t* x0 = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratchLock);
+        if(device >= 0 && (size_t)device < g_scratchAddr.size()) x0 = g_scratchAddr[device];
+    }
+    if(!x0) return (int)hipErrorNotInitialized;
+    const int e = dn_rc_encode(net, samples, n, x0, stream);
+    return e ? e : dn_rc_forward(net, x0, samples, n, pred, stream);
 }

@@ -399,6 +399,11 @@ class RadianceCache:
         if not self.lib.DNX_rc_predict_samples(self.ptr, samples.ptr, n, out.ptr):
             raise RuntimeError("DNX_rc_predict_samples failed")
 
+    def predict_samples_noalloc(self, samples: DeviceBuffer, n: int, out: DeviceBuffer) -> bool:
+        """The allocation-free variant (scratch rows in a fixed per-device array);
+        False when n exceeds DNX_RC_NOALLOC_MAX_SAMPLES or is not a multiple of 16."""
+        return bool(self.lib.DNX_rc_predict_samples_noalloc(self.ptr, samples.ptr, n, out.ptr))
+
     def set_params(self, weights: np.ndarray, table: np.ndarray) -> None:
         w = np.ascontiguousarray(weights, np.float32)
         t = np.ascontiguousarray(table, np.float32)

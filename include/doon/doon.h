@@ -469,6 +469,18 @@ bool DNX_rc_predict_requests(DNXradianceCache* rc, float* out, size_t requestCap
  * squared-error loss accumulate until an Adam step (`step`) or DNX_rc_set_params. */
 bool DNX_rc_train_samples(DNXradianceCache* rc, const float* samples, uint32_t n, bool step);
 bool DNX_rc_predict_samples(DNXradianceCache* rc, const float* samples, uint32_t n, float* out /* device float4 */);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): DNX_rc_predict_samples with the same inputs and outputs
+ * and the same results, but calling no hipMalloc, hipFree or allocating
+ * routine. Its encoded input rows live in a fixed per-device array of
+ * DNX_RC_NOALLOC_MAX_SAMPLES rows (24 MiB, held by each device until the
+ * process ends), set up by DN_init: n above that maximum is rejected (false,
+ * ERROR message). Every call on a device shares that array, so calls that
+ * overlap on one device (other streams, other threads) race. */
+#define DNX_RC_NOALLOC_MAX_SAMPLES 262144u
+bool DNX_rc_predict_samples_noalloc(DNXradianceCache* rc, const float* samples, uint32_t n,
+                                    float* out /* device float4 */);
 /* Host copies of the fp32 master parameters (weights: 12288 floats, the padded
  * row-major W0 64x48, W1 64x64, W2 64x64, W3 16x64; table: 16 x 2^log2Table x 2).
  * set_params also clears the gradients and the Adam state. */
