@@ -19,6 +19,7 @@ roofline (algorithmic bytes / HIP-event duration vs 8 TB/s) and a CPU baseline
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -33,16 +34,39 @@ VISIBLE_CAP = 32768
 PROFILE = ROOT / "profiles" / "frame_kernels_512x256x512_1920x1080.md"
 
 
-def profiled_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary of
-    this workload (FETCH_SIZE/WRITE_SIZE, scaled by a plain copy), or None."""
+def source_stamp() -> str:
+    """First 16 hex digits of the SHA-256 of the library's device and host
+    sources (doonengine_amd/csrc), in name order: the build a profile was
+    taken of."""
+    h = hashlib.sha256()
+    for f in sorted((ROOT / "doonengine_amd" / "csrc").iterdir()):
+        if f.suffix in (".hip", ".hpp", ".h", ".cpp"):
+            h.update(f.name.encode())
+            h.update(f.read_bytes())
+    return h.hexdigest()[:16]
+
+
+def profile_values() -> dict:
+    """The `key: value` lines of the committed rocprofv3 summary of this
+    workload (HBM bytes from FETCH_SIZE/WRITE_SIZE scaled by a plain copy,
+    VALU counters), or {} when the summary was taken of other sources than
+    the ones being benched."""
+    vals = {}
     try:
         for line in PROFILE.read_text().splitlines():
-            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
-                return float(line.split(":")[1])
+            if ": " in line and not line.startswith(("|", "#", " ", "`")):
+                k, v = line.split(": ", 1)
+                vals[k.strip()] = v.strip()
     except OSError:
-        pass
-    return None
+        return {}
+    return vals if vals.get("source_sha16") == source_stamp() else {}
+
+
+def fnum(vals: dict, key: str):
+    try:
+        return float(vals[key])
+    except (KeyError, ValueError):
+        return None
 
 
 def algorithmic_bytes(c: dict) -> float:
@@ -105,18 +129,24 @@ def cpu_baseline(vol, sc, budget_s: float) -> dict:
 
 
 def bench_batch(args) -> int:
-    """Batched uneven workload: `--scenes` seeded volumes (chunk grids U[4,48]
+    print(json.dumps(run_batch(args.scenes, args.steps, args.warmup)), flush=True)
+    return 0
+
+
+def run_batch(n_scenes: int, steps: int, warmup: int) -> dict:
+    """Batched uneven workload: `n_scenes` seeded volumes (chunk grids U[4,48]
     per axis, occupancy U[5%,60%], 2-16 materials, 10-40000 visible chunks),
     lit per frame either by one batched persistent graph launch (autotuned
     work split) or by one DN_update_lighting per volume. Both paths give
-    bit-identical state (tests/test_gpu_batch.py); here they are timed on
-    alternating frames of the same volumes."""
+    bit-identical state (tests/test_gpu_batch.py, and every volume against the
+    oracle in tests/test_gpu_configs.py); here they are timed on alternating
+    frames of the same volumes."""
     from doonengine_amd import _hip, scenes
     from doonengine_amd.volume import KIND_LIGHTING, Batch, Volume
 
     t_setup = time.perf_counter()
     vols, vis = [], []
-    for i in range(args.scenes):
+    for i in range(n_scenes):
         sc = scenes.batch_scene(i)
         v = Volume(sc.map_size, min_chunks=256)
         sc.apply(v)
@@ -140,8 +170,9 @@ def bench_batch(args) -> int:
 
     t = frame_sync()
     best, tune_ms = batch.autotune(1, 1000, t, repeats=2)
+    t_tuned = time.perf_counter()
     print(f"[bench] autotuned {best} ({tune_ms:.2f} ms compute-only)", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         batch.update_lighting(1, 1000, t)
         t = frame_sync()
     for v in vols:
@@ -149,7 +180,7 @@ def bench_batch(args) -> int:
     batch.timing(reset=True)
     tb = ts = 0.0
     lit_b = lit_s = 0
-    for k in range(2 * args.steps):
+    for k in range(2 * steps):
         t0 = time.perf_counter()
         if k % 2 == 0:
             batch.update_lighting(1, 1000, t)
@@ -165,26 +196,26 @@ def bench_batch(args) -> int:
             ts, lit_s = ts + dt, lit_s + lit
         t = frame_sync()
     ev_ms, launches = batch.timing(reset=True)
-    K = args.steps
+    K = steps
     result = {
         "metric": "voxel-lighting updates/s, batched uneven volumes (one persistent graph launch per frame)",
-        "value": lit_b / tb, "unit": "updates/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "value": lit_b / tb, "unit": "updates/s", "n_gpus": 1, "steps": K, "warmup": warmup,
         "ms_per_step": 1e3 * tb / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{args.scenes} seeded volumes, grids U[4,48]^3 chunks, occupancy U[5%,60%], "
+        "config": {"workload": f"{n_scenes} seeded volumes, grids U[4,48]^3 chunks, occupancy U[5%,60%], "
                                "2-16 materials, 10-40000 visible chunks, 16 direct/specular rays, 1 diffuse "
-                               "sample (<= 4 bounces)", "volumes": args.scenes, "parallelism": "single GPU"},
+                               "sample (<= 4 bounces)", "volumes": n_scenes, "parallelism": "single GPU"},
         "batched": {"ms_per_frame": 1e3 * tb / K, "graph_ms_per_frame": ev_ms / max(launches, 1),
                     "voxels_lit_per_frame": lit_b / K, "autotuned": best, "autotune_compute_ms": tune_ms},
         "per_volume_launches": {"ms_per_frame": 1e3 * ts / K, "voxels_lit_per_frame": lit_s / K,
                                 "updates_per_s": lit_s / ts},
         "speedup": (ts / max(lit_s, 1)) / (tb / max(lit_b, 1)),
+        "setup_s": t_tuned - t_setup,
     }
-    print(json.dumps(result), flush=True)
     batch.close()
     for v in vols:
         v.close()
-    return 0
+    return result
 
 
 def catmull_rom(points, u: float):
@@ -201,6 +232,11 @@ def catmull_rom(points, u: float):
 
 
 def bench_edits(args) -> int:
+    print(json.dumps(run_edits(args.frames or 600, args.warmup, args.edit_api)), flush=True)
+    return 0
+
+
+def run_edits(frames: int, warmup: int, edit_api: str) -> dict:
     """Dynamic edit streaming on the 16^3-chunk fBm map: each frame 256 seeded
     sphere carve/fill edits (radius U[3,24], centre uniform over the map,
     50/50, seed 4 + frame), then the reference's frame order: DN_draw (256x256,
@@ -225,23 +261,22 @@ def bench_edits(args) -> int:
     out = DeviceBuffer(W * H * 16)
     ctrl = [(1.0 + 14.0 * float(uniform01(k, 0, seed=4)), 7.0 + 6.0 * float(uniform01(k, 1, seed=4)),
              1.0 + 14.0 * float(uniform01(k, 2, seed=4))) for k in range(8)]
-    frames = args.frames or 600
     # the edit stream is input data: drawn for every frame before timing starts
     frame_edits = []
-    for f in range(frames + args.warmup):
+    for f in range(frames + warmup):
         fe = scenes.sphere_edits(sc, f)
-        frame_edits.append(Volume.sphere_edits(fe) if args.edit_api == "batch" else fe)
+        frame_edits.append(Volume.sphere_edits(fe) if edit_api == "batch" else fe)
     times, edit_ms, up_bytes, up_chunks, lit = [], [], [], [], []
     vol.counters(KIND_LIGHTING, reset=True)
     vol.lib.DNX_get_upload_stats(vol.ptr, None, None, None, True)
     b, c_, v_ = (ctypes_u64() for _ in range(3))
-    for f in range(frames + args.warmup):
+    for f in range(frames + warmup):
         pos, tan = catmull_rom(ctrl, f / max(frames, 1))
         vol.set_vec("camPos", tuple(float(x) for x in pos))
         yaw = float(np.degrees(np.arctan2(tan[0], tan[2])))
         vol.set_vec("camOrient", (25.0, yaw, 0.0))
         t0 = time.perf_counter()
-        if args.edit_api == "batch":
+        if edit_api == "batch":
             vol.shape_spheres(frame_edits[f])
         else:
             for mat, alb, ctr, r, flip in frame_edits[f]:
@@ -255,7 +290,7 @@ def bench_edits(args) -> int:
         t2 = time.perf_counter()
         vol.lib.DNX_get_upload_stats(vol.ptr, b, c_, v_, True)
         litf = vol.counters(KIND_LIGHTING, reset=True)["litVoxels"]
-        if f >= args.warmup:
+        if f >= warmup:
             times.append(t2 - t0)
             edit_ms.append(1e3 * (t1 - t0))
             up_bytes.append(b.value)
@@ -267,12 +302,12 @@ def bench_edits(args) -> int:
     result = {
         "metric": "dynamic edit streaming: voxel-lighting updates/s and frame time p50/p99",
         "value": float(np.sum(lit) / np.sum(t)), "unit": "updates/s", "n_gpus": 1, "steps": frames,
-        "warmup": args.warmup, "ms_per_step": float(1e3 * t.mean()), "higher_is_better": True, "scaling": "weak",
+        "warmup": warmup, "ms_per_step": float(1e3 * t.mean()), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "16x16x16 chunks (128^3 voxels) fBm ~35%, 256 sphere edits/frame (radius 3-24, "
                                "carve/fill 50/50), 256x256 draw, 16 direct/specular rays, 1 diffuse sample",
                    "frames": frames, "edit_api": "DNX_shape_spheres (device, one call per frame)"
-                   if args.edit_api == "batch" else "DN_shape_sphere (host, one call per edit)",
+                   if edit_api == "batch" else "DN_shape_sphere (host, one call per edit)",
                    "parallelism": "single GPU"},
         "frame_ms_p50": float(1e3 * np.percentile(t, 50)), "frame_ms_p99": float(1e3 * np.percentile(t, 99)),
         "edit_ms_p50": float(np.percentile(edit_ms, 50)),
@@ -280,9 +315,8 @@ def bench_edits(args) -> int:
         "dirty_fraction": float(np.mean(up_chunks) / (sc.map_size[0] * sc.map_size[1] * sc.map_size[2])),
         "voxels_lit_per_frame": float(np.mean(lit)),
     }
-    print(json.dumps(result), flush=True)
     vol.close()
-    return 0
+    return result
 
 
 def bench_cache(args) -> int:
@@ -452,6 +486,10 @@ def main() -> int:
     ap.add_argument("--cache-ema-growth", type=float, default=8.0,
                     help="cache parameter average: decay min(ema, 1 - G / step) while the window grows (0: off)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--ref-frames", type=int, default=5,
+                    help="frame workload: frames timed with the reference's own lighting settings (0: skip)")
+    ap.add_argument("--no-summaries", dest="summaries", action="store_false",
+                    help="frame workload: skip the short edit-streaming and batched-volume summaries")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="after the timed frames, copy BYTES device-to-device once (a plain copy kernel of known "
@@ -536,6 +574,39 @@ def main() -> int:
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
         cpu = cpu_baseline(vol, sc, args.cpu_seconds)
 
+    # The same frame with the reference's own lighting algorithm
+    # (voxelLighting.comp:207-288 at its defaults): one jittered shadow ray per
+    # diffuse sample instead of the fixed direct rays, the 15 spherePoints for
+    # specular rays, up to 5 diffuse bounces.
+    refset = None
+    if args.ref_frames > 0:
+        vol.v.numDirectRays = 0
+        vol.v.numSpecularRays = 15
+        vol.v.diffuseBounceLimit = 5
+        for _ in range(2):
+            step()
+        vol.synchronize()
+        vol.counters(KIND_DRAW, reset=True)
+        vol.counters(KIND_LIGHTING, reset=True)
+        vol.enable_timing(True)
+        vol.timing(reset=True)
+        t1 = time.perf_counter()
+        for _ in range(args.ref_frames):
+            step()
+        vol.synchronize()
+        e1 = time.perf_counter() - t1
+        vol.enable_timing(False)
+        tm1 = vol.timing(reset=True)
+        lc1 = vol.counters(KIND_LIGHTING)
+        lms1 = tm1["lightingMs"] / max(tm1["lightingLaunches"], 1)
+        refset = {"settings": "numDirectRays 0 (one jittered shadow ray per diffuse sample), 15 spherePoints "
+                              "specular rays, diffuseBounceLimit 5, 1 diffuse sample",
+                  "frames": args.ref_frames, "ms_per_frame": 1e3 * e1 / args.ref_frames, "lighting_ms": lms1,
+                  "voxel_lighting_updates_per_s": lc1["litVoxels"] / (lms1 * 1e-3 * args.ref_frames),
+                  "lighting_rays_per_s": lc1["rays"] / (lms1 * 1e-3 * args.ref_frames),
+                  "lighting_rays_per_update": lc1["rays"] / max(lc1["litVoxels"], 1)}
+        print(f"[bench] reference settings: lighting {lms1:.3f} ms/launch", file=sys.stderr, flush=True)
+
     result = {
         "metric": "DDA rays/s & voxel-lighting updates/s; cache vs plain frames-to-1%-RMSE",
         "value": rays / elapsed,
@@ -568,15 +639,55 @@ def main() -> int:
         "kernel_ms": {"lighting": light_ms, "draw": tm["drawMs"] / max(tm["drawLaunches"], 1),
                       "visible": tm["visibleMs"] / max(tm["visibleLaunches"], 1),
                       "epilogue": tm["epilogueMs"] / max(tm["epilogueLaunches"], 1)},
-        "roofline": {"kernel": "lighting pass (k_direct + k_lighting)", "bound": "hbm", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic("lighting_pass") or profiled_traffic("k_lighting"),
-                     "algorithmic_bytes_per_launch": light_bytes},
+        "roofline": roofline(light_bytes, light_ms, achieved),
         "cpu_baseline": cpu,
+        "reference_settings": refset,
     }
-    print(json.dumps(result), flush=True)
     vol.close()
+    # short summaries of the edit-streaming and batched-volume workloads
+    if args.summaries:
+        ed = run_edits(60, 5, "batch")
+        result["edits_summary"] = {k: ed[k] for k in ("value", "unit", "frame_ms_p50", "frame_ms_p99", "edit_ms_p50",
+                                                       "upload_bytes_per_frame", "voxels_lit_per_frame")}
+        result["edits_summary"]["workload"] = ed["config"]["workload"] + ", 60 frames"
+        print(f"[bench] edits summary: frame p50 {ed['frame_ms_p50']:.2f} ms", file=sys.stderr, flush=True)
+        bt = run_batch(384, 3, 1)
+        result["batch_summary"] = {"value": bt["value"], "unit": bt["unit"], "workload": bt["config"]["workload"],
+                                   "batched_ms_per_frame": bt["batched"]["ms_per_frame"],
+                                   "per_volume_ms_per_frame": bt["per_volume_launches"]["ms_per_frame"],
+                                   "speedup": bt["speedup"], "voxels_lit_per_frame": bt["batched"]["voxels_lit_per_frame"],
+                                   "frames": 3}
+        print(f"[bench] batch summary: {bt['batched']['ms_per_frame']:.1f} ms/frame", file=sys.stderr, flush=True)
+    print(json.dumps(result), flush=True)
     return 0
+
+
+def roofline(light_bytes: float, light_ms: float, achieved: float) -> dict:
+    """The lighting pass against the HBM roofline on the traversal model's
+    algorithmic bytes, with what the committed profile of this build measured
+    beside it: the real HBM traffic (so the fraction of HBM bandwidth actually
+    used) and the VALU counters that show what bounds the kernels."""
+    pv = profile_values()
+    traffic = fnum(pv, "traffic_bytes_per_launch lighting_pass")
+    out = {"kernel": "lighting pass (k_direct + k_lighting)", "bound": "hbm", "achieved": achieved,
+           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+           "algorithmic_bytes_per_launch": light_bytes,
+           "profile_source_sha16": pv.get("source_sha16"), "source_sha16": source_stamp()}
+    if traffic is not None:
+        measured = traffic / (light_ms * 1e-3) / 1e9
+        out["measured_hbm_gbs"] = measured
+        out["measured_hbm_frac"] = measured / HBM_PEAK_GBS
+    valu = {}
+    for k in ("k_direct", "k_lighting"):
+        busy, lane = fnum(pv, f"valu_busy {k}"), fnum(pv, f"lane_use {k}")
+        if busy is not None:
+            valu[k] = {"valu_busy": busy, "lane_use": lane, "valu_insts": fnum(pv, f"valu_insts {k}"),
+                       "valu_lane_insts_per_dda_step": fnum(pv, f"valu_lane_insts_per_step {k}")}
+    if valu:
+        out["limiter"] = ("VALU issue: the kernels' VALU pipes are busy most of the time while HBM traffic "
+                          "stays far below peak (the scene stays in L2); lanes idle on rays that end early")
+        out["valu"] = valu
+    return out
 
 
 if __name__ == "__main__":

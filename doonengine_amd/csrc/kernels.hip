@@ -111,11 +111,22 @@ void k_draw(DNdrawParams P)
     c.maskRow = maskRows + threadIdx.x * DN_MASK_ROW;
     const uint32_t qx = (P.width + 7u) / 8u, nq = qx * ((P.height + 7u) / 8u);
     uint32_t drawn = 0;
-    for(;;)
+    // The first 3/4 of the quads are dealt out statically (wave w takes quads
+    // w, w + W, w + 2W, ...) and the rest from the device counter, in order,
+    // to even out the waves' ends. Every quad from the counter costs one
+    // atomic on that single word, which all 8 XCDs share; at one atomic per
+    // quad those serialise to a floor of about 0.5 ms per 1920x1080 frame
+    // (0.70 ms measured against 0.52 ms with the static share).
+    const uint32_t waves = gridDim.x * 4u, wave = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t perWave = (nq / 4u * 3u) / waves, head = perWave * waves;
+    for(uint32_t k = 0;; k++)
     {
-        uint32_t q = 0;
-        if(lane == 0) q = atomicAdd(P.quadCounter, 1u);
-        q = __shfl(q, 0, 64);
+        uint32_t q = wave + k * waves;
+        if(k >= perWave)
+        {
+            if(lane == 0) q = atomicAdd(P.quadCounter, 1u);
+            q = __shfl(q, 0, 64) + head;
+        }
         if(q >= nq) break;
         const uint32_t x = (q % qx) * 8u + (lane & 7u), y = (q / qx) * 8u + (lane >> 3);
         if(x < P.width && y < P.height)
