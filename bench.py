@@ -34,15 +34,18 @@ VISIBLE_CAP = 32768
 PROFILE = ROOT / "profiles" / "frame_kernels_512x256x512_1920x1080.md"
 
 
+FRAME_SOURCES = ("dn_device.hpp", "dn_internal.h", "dn_light.hpp", "engine.cpp", "kernels.hip")
+
+
 def source_stamp() -> str:
-    """First 16 hex digits of the SHA-256 of the library's device and host
-    sources (doonengine_amd/csrc), in name order: the build a profile was
-    taken of."""
+    """First 16 hex digits of the SHA-256 of the sources the frame's kernels
+    and their launches are built from (doonengine_amd/csrc), in name order:
+    the build a profile was taken of."""
     h = hashlib.sha256()
-    for f in sorted((ROOT / "doonengine_amd" / "csrc").iterdir()):
-        if f.suffix in (".hip", ".hpp", ".h", ".cpp"):
-            h.update(f.name.encode())
-            h.update(f.read_bytes())
+    for name in FRAME_SOURCES:
+        f = ROOT / "doonengine_amd" / "csrc" / name
+        h.update(name.encode())
+        h.update(f.read_bytes())
     return h.hexdigest()[:16]
 
 
@@ -681,9 +684,9 @@ def roofline(light_bytes: float, light_ms: float, achieved: float) -> dict:
     for k in ("k_direct", "k_lighting"):
         busy, lane = fnum(pv, f"valu_busy {k}"), fnum(pv, f"lane_use {k}")
         if busy is not None:
-            valu[k] = {"valu_busy": busy, "lane_use": lane, "valu_insts": fnum(pv, f"valu_insts {k}"),
-                       "valu_lane_insts_per_dda_step": fnum(pv, f"valu_lane_insts_per_step {k}")}
+            valu[k] = {"valu_busy": busy, "lane_use": lane, "valu_insts": fnum(pv, f"valu_insts {k}")}
     if valu:
+        valu["lane_insts_per_dda_trip"] = fnum(pv, "valu_lane_insts_per_step lighting_pass")
         out["limiter"] = ("VALU issue: the kernels' VALU pipes are busy most of the time while HBM traffic "
                           "stays far below peak (the scene stays in L2); lanes idle on rays that end early")
         out["valu"] = valu

@@ -283,11 +283,12 @@ bool alloc_tile_buffers(DNgpuVolume* g, uint32_t tiles)
               dalloc(&g->dRequests, (size_t)tiles * 16u, "lighting request buffer") &&
               dalloc(&g->dBlockSums, g->visBlocks, "request scan") && dalloc(&g->dReqList, tiles, "request list");
     if(!ok) return false;
-    hipMemsetAsync(g->dFlags, 0, sizeof(uint32_t) * tiles, g->stream);
-    hipMemsetAsync(g->dLastUsed, 0, sizeof(uint32_t) * tiles, g->stream);
-    hipMemsetAsync(g->dChunks, 0, sizeof(DNchunkRec) * tiles, g->stream);
+    ok = HIP_OK(hipMemsetAsync(g->dFlags, 0, sizeof(uint32_t) * tiles, g->stream), "map buffer clear") &&
+         HIP_OK(hipMemsetAsync(g->dLastUsed, 0, sizeof(uint32_t) * tiles, g->stream), "map buffer clear") &&
+         HIP_OK(hipMemsetAsync(g->dChunks, 0, sizeof(DNchunkRec) * tiles, g->stream), "chunk buffer clear");
     for(uint32_t* b : {g->dOccLoaded, g->dOccPresent, g->dFresh, g->dLitBits, g->dVisProp})
-        hipMemsetAsync(b, 0, sizeof(uint32_t) * g->occWords, g->stream);
+        ok = ok && HIP_OK(hipMemsetAsync(b, 0, sizeof(uint32_t) * g->occWords, g->stream), "occupancy clear");
+    if(!ok) return false;
     g->state.assign(tiles, 0);
     g->nodeStart.assign(tiles, 0);
     g->nodeClass.assign(tiles, -1);
@@ -787,10 +788,14 @@ extern "C" DNvolume* DN_create_volume(DNuvec3 mapSize, unsigned int minChunks)
         DN_delete_volume(vol);
         return nullptr;
     }
-    hipMemsetAsync(g->dNumRequests, 0, sizeof(uint32_t), g->stream);
-    hipMemsetAsync(g->dWorkCounter, 0, 6 * sizeof(uint32_t), g->stream);
-    hipMemsetAsync(g->dCounters, 0, 3 * CNT_SLOTS * sizeof(uint64_t), g->stream);
-    hipMemsetAsync(g->dMats, 0, sizeof(DNmat) * DN_MAX_MATERIALS, g->stream);
+    if(!HIP_OK(hipMemsetAsync(g->dNumRequests, 0, sizeof(uint32_t), g->stream), "request count") ||
+       !HIP_OK(hipMemsetAsync(g->dWorkCounter, 0, 6 * sizeof(uint32_t), g->stream), "work counter") ||
+       !HIP_OK(hipMemsetAsync(g->dCounters, 0, 3 * CNT_SLOTS * sizeof(uint64_t), g->stream), "counters") ||
+       !HIP_OK(hipMemsetAsync(g->dMats, 0, sizeof(DNmat) * DN_MAX_MATERIALS, g->stream), "material buffer"))
+    {
+        DN_delete_volume(vol);
+        return nullptr;
+    }
 
     size_t numChunks = std::min<size_t>(tiles, minChunks);
     if(numChunks < 1) numChunks = 1;
@@ -2523,11 +2528,15 @@ extern "C" bool DNX_synchronize(DNvolume* vol)
 extern "C" size_t DNX_fetch_lighting_requests(DNvolume* vol)
 {
     DNgpuVolume* g = vol->gpu;
-    hipMemcpyAsync(g->hSmall, g->dNumRequests, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream);
-    if(!HIP_OK(hipStreamSynchronize(g->stream), "request readback")) return 0;
+    if(!HIP_OK(hipMemcpyAsync(g->hSmall, g->dNumRequests, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream),
+               "request readback") ||
+       !HIP_OK(hipStreamSynchronize(g->stream), "request readback"))
+        return 0;
     size_t n = g->hSmall[0];
     if(n > vol->lightingRequestCap && !DN_set_max_lighting_requests(vol, n)) return 0;
-    if(n) hipMemcpy(vol->lightingRequests, g->dRequests, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
+    if(n && !HIP_OK(hipMemcpy(vol->lightingRequests, g->dRequests, sizeof(uint32_t) * n, hipMemcpyDeviceToHost),
+                    "request readback"))
+        return 0;
     vol->numLightingRequests = n;
     return n;
 }
@@ -2542,9 +2551,10 @@ extern "C" size_t DNX_export_gpu_state(DNvolume* vol, uint32_t* map, uint32_t* c
     {
         std::vector<uint32_t> fl(T), lu(T);
         std::vector<DNchunkRec> recs(T);
-        hipMemcpy(fl.data(), g->dFlags, sizeof(uint32_t) * T, hipMemcpyDeviceToHost);
-        hipMemcpy(lu.data(), g->dLastUsed, sizeof(uint32_t) * T, hipMemcpyDeviceToHost);
-        hipMemcpy(recs.data(), g->dChunks, sizeof(DNchunkRec) * T, hipMemcpyDeviceToHost);
+        if(!HIP_OK(hipMemcpy(fl.data(), g->dFlags, sizeof(uint32_t) * T, hipMemcpyDeviceToHost), "export") ||
+           !HIP_OK(hipMemcpy(lu.data(), g->dLastUsed, sizeof(uint32_t) * T, hipMemcpyDeviceToHost), "export") ||
+           !HIP_OK(hipMemcpy(recs.data(), g->dChunks, sizeof(DNchunkRec) * T, hipMemcpyDeviceToHost), "export"))
+            return 0;
         for(uint32_t t = 0; t < T; t++)
         {
             bool loaded = (fl[t] & 3u) == 2u;
@@ -2571,7 +2581,8 @@ extern "C" size_t DNX_export_gpu_state(DNvolume* vol, uint32_t* map, uint32_t* c
     if(voxels && voxelWords)
     {
         size_t words = std::min(voxelWords, (size_t)g->pool.cap * 4);
-        hipMemcpy(voxels, g->dVoxels, sizeof(uint32_t) * words, hipMemcpyDeviceToHost);
+        if(!HIP_OK(hipMemcpy(voxels, g->dVoxels, sizeof(uint32_t) * words, hipMemcpyDeviceToHost), "export"))
+            return 0;
     }
     return g->pool.cap;
 }
@@ -2590,8 +2601,12 @@ extern "C" void DNX_get_counters(DNvolume* vol, int kind, DNXcounters* out, bool
     DNgpuVolume* g = vol->gpu;
     if(kind < 0 || kind > 2) { memset(out, 0, sizeof(*out)); return; }
     uint64_t c[CNT_SLOTS];
-    hipStreamSynchronize(g->stream);
-    hipMemcpy(c, g->dCounters + kind * CNT_SLOTS, sizeof(c), hipMemcpyDeviceToHost);
+    if(!HIP_OK(hipStreamSynchronize(g->stream), "counters") ||
+       !HIP_OK(hipMemcpy(c, g->dCounters + kind * CNT_SLOTS, sizeof(c), hipMemcpyDeviceToHost), "counters"))
+    {
+        memset(out, 0, sizeof(*out));
+        return;
+    }
     out->rays = c[CNT_RAYS];
     out->mapSteps = c[CNT_MAPSTEPS];
     out->voxelSteps = c[CNT_VOXSTEPS];
@@ -2903,11 +2918,12 @@ extern "C" bool DNX_batch_update_lighting(DNXbatch* b, int numDiffuseSamples, in
         bool ok = HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed), "batch capture");
         if(ok)
         {
-            hipMemcpyAsync(b->dParams, b->hParams[s], sizeof(DNlightParams) * b->vols.size(), hipMemcpyHostToDevice,
-                           cs);
+            const bool copied = HIP_OK(hipMemcpyAsync(b->dParams, b->hParams[s], sizeof(DNlightParams) * b->vols.size(),
+                                                      hipMemcpyHostToDevice, cs), "batch parameters");
             int le = dn_launch_batch(batch_params(b, false), c.blockThreads, c.blocksPerCU, maxTiles, maxReq, true,
                                      b->anyDirect, cs);
-            ok = HIP_OK(hipStreamEndCapture(cs, &graph), "batch capture") && le == 0;
+            // the capture is ended whatever failed inside it
+            ok = HIP_OK(hipStreamEndCapture(cs, &graph), "batch capture") && le == 0 && copied;
             if(ok) ok = HIP_OK(hipGraphInstantiate(&b->exec[s], graph, nullptr, nullptr, 0), "batch graph");
             if(graph) hipGraphDestroy(graph);
         }

@@ -6,7 +6,9 @@
 // index where (N_l + 2)^3 entries fit the table, else the spatial hash
 // (x * 1 ^ y * 2654435761 ^ z * 805459861) mod T.
 // Network: 4 weight matrices without biases, ReLU between them; operands in
-// bfloat16 on v_mfma_f32_16x16x16_bf16 with FP32 accumulation. Layers are
+// bfloat16 on v_mfma_f32_16x16x32_bf16 (v_mfma_f32_16x16x16_bf16 for a
+// lone 16-wide K block and for the weight-gradient GEMM) with FP32
+// accumulation. Layers are
 // computed transposed (D[feature][sample] = W * X^T), so one layer's 16x16
 // accumulator tile is, lane for lane, the next layer's B operand: a lane holds
 // sample (lane & 15) and features 4 * (lane >> 4) + 0..3 of a 16-feature block.
@@ -33,9 +35,25 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 }
 __device__ __forceinline__ float bf2f(uint32_t h) { return __uint_as_float(h << 16); }
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+
 __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// Two consecutive K = 16 steps as one v_mfma_f32_16x16x32_bf16 (gfx950):
+// c + A0 B0 + A1 B1. A lane group's 8 k-values of A are its 4 of A0 then its
+// 4 of A1, and likewise for B, so A and B pair the same 32 k-values (a
+// permutation of the K = 32 block, which leaves the dot product a sum over
+// the same terms) and two C-layout tiles feed B without any shuffle.
+__device__ __forceinline__ f32x4 mfma32(bf16x4 a0, bf16x4 a1, bf16x4 b0, bf16x4 b1, f32x4 c)
+{
+    const s16x8 a = {a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
+    const s16x8 b = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                   0, 0);
 }
 
 __device__ __forceinline__ bf16x4 ld_bf16x4(const uint16_t* p)
@@ -184,12 +202,18 @@ __device__ __forceinline__ void wave_add_pairs(float* gt, uint32_t key, float v0
 
 // Hash-table gradient: each wave takes 64 consecutive samples at one level
 // and spreads the level's two feature gradients over the 8 corners by their
-// weights.
+// weights. The FP32 atomics bound it: without the per-wave sums of the 8
+// coarsest levels it takes 1224 us per 2^17 samples, with them at 4 levels 961,
+// 6 levels 870, 8 levels 820.
 __global__ __launch_bounds__(256) void k_rc_table_grad(DNrcNet net, const float* samples, uint32_t n, const float* denc)
 {
     const uint32_t g = blockIdx.x * 256u + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u, w = g >> 6;
-    const uint32_t l = w & 15u, s = (w >> 4) * 64u + lane;
+    // level-major: the waves running at a time work on one level's gradient
+    // table (4 MiB at 2^19 entries) instead of all sixteen (820 against 840 us
+    // per 2^17 samples); finest level first. Waves past the last level idle.
+    const uint32_t sbs = (n + 63u) / 64u, lvl = w / sbs;
+    const uint32_t l = 15u - min(lvl, 15u), s = lvl < 16u ? (w % sbs) * 64u + lane : n;
     float d0 = 0.0f, d1 = 0.0f;
     if(s < n)
     {
@@ -253,9 +277,9 @@ __global__ __launch_bounds__(256) void k_rc_mlp(DNrcNet net, const uint16_t* x0,
         f32x4 acc[4];
         for(uint32_t mb = 0; mb < 4; mb++)
         {
-            acc[mb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-            for(uint32_t ks = 0; ks < 3; ks++)
-                acc[mb] = mfma16(a_rows(W + RC_OFF_W0, 48, mb * 16u, ks * 16u, lane), xin[ks], acc[mb]);
+            acc[mb] = mfma32(a_rows(W + RC_OFF_W0, 48, mb * 16u, 0u, lane), a_rows(W + RC_OFF_W0, 48, mb * 16u, 16u, lane),
+                             xin[0], xin[1], f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+            acc[mb] = mfma16(a_rows(W + RC_OFF_W0, 48, mb * 16u, 32u, lane), xin[2], acc[mb]);
         }
         for(uint32_t mb = 0; mb < 4; mb++) h[0][mb] = relu_bf16(acc[mb]);
         for(uint32_t layer = 1; layer < 3; layer++)
@@ -264,13 +288,16 @@ __global__ __launch_bounds__(256) void k_rc_mlp(DNrcNet net, const uint16_t* x0,
             for(uint32_t mb = 0; mb < 4; mb++)
             {
                 acc[mb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-                for(uint32_t ks = 0; ks < 4; ks++)
-                    acc[mb] = mfma16(a_rows(Wl, 64, mb * 16u, ks * 16u, lane), h[layer - 1][ks], acc[mb]);
+                for(uint32_t ks = 0; ks < 4; ks += 2)
+                    acc[mb] = mfma32(a_rows(Wl, 64, mb * 16u, ks * 16u, lane), a_rows(Wl, 64, mb * 16u, ks * 16u + 16u, lane),
+                                     h[layer - 1][ks], h[layer - 1][ks + 1], acc[mb]);
             }
             for(uint32_t mb = 0; mb < 4; mb++) h[layer][mb] = relu_bf16(acc[mb]);
         }
         f32x4 y = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        for(uint32_t ks = 0; ks < 4; ks++) y = mfma16(a_rows(W + RC_OFF_W3, 64, 0, ks * 16u, lane), h[2][ks], y);
+        for(uint32_t ks = 0; ks < 4; ks += 2)
+            y = mfma32(a_rows(W + RC_OFF_W3, 64, 0, ks * 16u, lane), a_rows(W + RC_OFF_W3, 64, 0, ks * 16u + 16u, lane),
+                       h[2][ks], h[2][ks + 1], y);
         // y: lane < 16 holds outputs 0..3 of sample s0 + lane (3 used)
         if(!TRAIN)
         {
@@ -314,7 +341,9 @@ __global__ __launch_bounds__(256) void k_rc_mlp(DNrcNet net, const uint16_t* x0,
             for(uint32_t mb = 0; mb < 4; mb++)
             {
                 f32x4 d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-                for(uint32_t ks = 0; ks < 4; ks++) d = mfma16(a_rows(WTl, 64, mb * 16u, ks * 16u, lane), dz[ks], d);
+                for(uint32_t ks = 0; ks < 4; ks += 2)
+                    d = mfma32(a_rows(WTl, 64, mb * 16u, ks * 16u, lane), a_rows(WTl, 64, mb * 16u, ks * 16u + 16u, lane),
+                               dz[ks], dz[ks + 1], d);
                 nz[mb] = relu_grad(d, h[layer][mb]);
                 store_T(acts.dzT[layer], RC_W, tile, mb, nz[mb], lane);
             }
@@ -324,15 +353,19 @@ __global__ __launch_bounds__(256) void k_rc_mlp(DNrcNet net, const uint16_t* x0,
         for(uint32_t mb = 0; mb < 2; mb++)
         {
             f32x4 d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-            for(uint32_t ks = 0; ks < 4; ks++) d = mfma16(a_rows(WT + RC_OFF_W0, 64, mb * 16u, ks * 16u, lane), dz[ks], d);
+            for(uint32_t ks = 0; ks < 4; ks += 2)
+                d = mfma32(a_rows(WT + RC_OFF_W0, 64, mb * 16u, ks * 16u, lane),
+                           a_rows(WT + RC_OFF_W0, 64, mb * 16u, ks * 16u + 16u, lane), dz[ks], dz[ks + 1], d);
             reinterpret_cast<float4*>(acts.denc + (size_t)sm * RC_ENC + mb * 16u + 4u * (lane >> 4))[0] =
                 make_float4(d[0], d[1], d[2], d[3]);
         }
     }
 }
 
-// Weight gradients dW = dZ^T X summed over all tiles: blocks take contiguous
-// tile ranges; wave w owns output rows 16w..16w+15 of W0..W2 and output
+// Weight gradients dW = dZ^T X summed over all tiles (K = samples, one
+// 16-sample tile per v_mfma_f32_16x16x16_bf16: pairing two tiles into one
+// K = 32 step doubles the scattered 8-byte operand loads per step and measured
+// 153 against 69 us per 2^17 samples); blocks take contiguous tile ranges; wave w owns output rows 16w..16w+15 of W0..W2 and output
 // column block w of W3; partial sums go out by FP32 atomics.
 __global__ __launch_bounds__(256) void k_rc_wgrad(DNrcNet net, uint32_t tiles, DNrcActs acts)
 {

@@ -2,7 +2,10 @@
 Mesa llvmpipe (LP_NUM_THREADS threads), on the 128^3 and 512x256x512 scenes.
 
 Ray and update counts come from the FP32 oracle on the same frame (llvmpipe
-cannot count). Prints one JSON object. Needs the reference tree (shaders are
+cannot count). Every time is the median of 10 frames after a warm-up frame
+(shader JIT). The algorithm is the shaders' own, at their defaults: 15
+specular rays, one jittered shadow ray per diffuse sample, <= 5 diffuse
+bounces, 1 sample. Prints one JSON object. Needs the reference tree (shaders are
 read from it at run time) and Mesa; run where both exist:
     LP_NUM_THREADS=8 python oracle/time_reference.py
 """
@@ -28,7 +31,7 @@ from helpers import build_state, camera  # noqa: E402
 SUN = np.array([-1.0, 1.0, -1.0], np.float32) * (np.float32(1.0) / np.sqrt(np.float32(3.0)))
 
 
-def run(ref, o, sc, width, height, max_requests, frames=3, bounces=5):
+def run(ref, o, sc, width, height, max_requests, frames=10, bounces=5):
     m, c, v, mats = build_state(sc)
     _, _, iv, ivp = camera(sc, width, height)
     s = ref.scene(sc.map_size, m, c, v, mats)
@@ -71,12 +74,13 @@ def main():
     threads = int(os.environ.get("LP_NUM_THREADS", os.cpu_count() or 1))
     ref = Llvmpipe(instrument=False)
     o = oracle.load("f32")
-    out = {"renderer": ref.renderer, "threads": threads, "host_cpus": os.cpu_count(),
+    cpu = next((l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")), "")
+    out = {"renderer": ref.renderer, "threads": threads, "host_cpus": os.cpu_count(), "cpu": cpu, "frames": 10,
            "algorithm": "reference shaders as written: 15 specular rays, 1 jittered shadow ray per sample, "
                         "<= 5 diffuse bounces, 1 sample"}
     t0 = time.time()
     out["config1"] = run(ref, o, scenes.config1(reference_like=True), 256, 256, max_requests=1 << 30)
-    out["config2_prefix4096"] = run(ref, o, scenes.config2(), 1920, 1080, max_requests=4096, frames=2, bounces=4)
+    out["config2_prefix4096"] = run(ref, o, scenes.config2(), 1920, 1080, max_requests=4096)
     out["wall_s"] = time.time() - t0
     print(json.dumps(out))
 
