@@ -12,9 +12,18 @@ empty face neighbour; 16201 in this scene) visible each frame, a full relight of
 everything that can be seen from somewhere.
 
 Prints ONE JSON line. `value` is DDA rays/s (step_map calls of draw + lighting
-per second); voxel-lighting updates/s, per-kernel times, the lighting kernel's
-roofline (algorithmic bytes / HIP-event duration vs 8 TB/s) and a CPU baseline
-(the oracle on a bounded sample, on this host) ride along.
+per second); beside it:
+- voxel-lighting updates/s, per-frame ray and step counts, per-kernel times;
+- `roofline`: the lighting pass's algorithmic bytes / HIP-event duration against
+  8 TB/s, and, from the committed rocprofv3 summary when it was taken of the
+  same frame sources, its measured HBM traffic and fraction and the VALU
+  counters that bound it;
+- `cpu_baseline`: the FP32 oracle on a bounded sample of the frame, on this host;
+- `reference_settings`: the same frame lit with the reference's own lighting
+  algorithm (one jittered shadow ray per sample, 15 specular rays, <= 5
+  bounces), the like-for-like figure for the llvmpipe baseline;
+- `edits_summary` / `batch_summary`: 60 frames of the edit-streaming workload
+  and 3 frames of the 384-volume batch (`--workload edits|batch` for the full runs).
 """
 from __future__ import annotations
 

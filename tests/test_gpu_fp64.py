@@ -1,7 +1,8 @@
 """GPU lighting against the FP64 oracle (same algorithm in double precision).
 
 Deterministic terms (ambient + fixed direct rays with the diffuse path off,
-and specular with the fixed sphere points) agree within 1e-4 relative.
+and specular with the fixed sphere points) agree within 1e-4 relative on
+every voxel.
 Diffuse path tracing is compared over N frames: the GPU's mean of its
 per-frame samples against the FP64 oracle's, per voxel, inside a 3-sigma
 Monte-Carlo band, sigma being the voxel's FP64 per-frame sample standard
@@ -71,7 +72,8 @@ def test_deterministic_terms_within_1e4(gpu):
     print(f"[fp64] deterministic: {valid.sum()} voxels ({spec_vox} with specular), rel err median "
           f"{np.median(per_voxel):.2e} max {per_voxel.max():.2e}, within 1e-4: {frac_ok:.5f}", flush=True)
     assert spec_vox > 50
-    assert frac_ok >= 0.999          # the rest: FP32/FP64 DDA decisions that differ on grazing rays
+    # every voxel (measured max 3.4e-7): no DDA decision of these rays differs between FP32 and FP64
+    assert frac_ok == 1.0
     vol.close()
 
 
