@@ -6,8 +6,8 @@
   pre-quantisation values, flags, sample counts, ray counters).
 - 64x32x64 chunks (512x256x512 voxels, fBm + 2000 boxes), 1920x1080 draw,
   lighting of every chunk the draw sees plus up to 32768 exposed chunks, 32
-  direct and specular rays, <= 4 diffuse bounces: one whole frame, compared the
-  same way (the oracle's passes split over host threads, which gives the
+  direct and specular rays, <= 4 diffuse bounces: two whole frames (the first
+  sample's deterministic directions, then random ones), compared the same way (the oracle's passes split over host threads, which gives the
   single-thread result exactly).
 - Edit streaming on the 16^3-chunk map: two frames of 256 seeded sphere carves
   and fills of radius 3-24 through DNX_shape_spheres, against the oracle's
@@ -92,10 +92,12 @@ def test_c2_full_frame(c2_volume):
     vol, sc = c2_volume
     o = oracle.load("f32")
     assert (sc.width, sc.height) == (1920, 1080)
-    nreq, nlit = run_frame_and_compare(vol, sc, o, 0, 123.456, sc.width, sc.height, threads=THREADS,
-                                       mark_visible=32768, full_draw=True)
-    print(f"[c2] {nreq} requests, {nlit} voxels lit, all bit-exact", flush=True)
-    assert nreq > 100000 and nlit > 3_000_000
+    # frame 1: every voxel's first sample (deterministic directions); frame 2: the frame-uniform random ones
+    for f in range(2):
+        nreq, nlit = run_frame_and_compare(vol, sc, o, f, 123.456 + f / 60.0, sc.width, sc.height, threads=THREADS,
+                                           mark_visible=32768, full_draw=True)
+        print(f"[c2] frame {f}: {nreq} requests, {nlit} voxels lit, all bit-exact", flush=True)
+        assert nreq > 100000 and nlit > 3_000_000
 
 
 def oracle_upload(o, grid, mats, t, map_size):
