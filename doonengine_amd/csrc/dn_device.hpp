@@ -443,6 +443,16 @@ __device__ bool step_map(const DNsceneView& s, RayCtx& c, f3& dir, f3 inv, f3& p
         if(FINITE) iterate_dda_finite(d, hitNormal);
         else iterate_dda(d, hitNormal);
         ignoreFirst = false;
+        // A lighting ray beyond the box of present tiles on the side it moves
+        // toward (a positive step past presHi, any other step below presLo)
+        // meets no loaded or unloaded tile before it leaves the map, so the
+        // rest of the walk would only count trips: it ends here as the same
+        // miss, with the same colour terms. Only the trip counters differ from
+        // the reference walk's; the oracle applies the same rule.
+        if(!REFRACT && ((d.sx > 0 ? d.px > s.presHi[0] : d.px < s.presLo[0]) ||
+                        (d.sy > 0 ? d.py > s.presHi[1] : d.py < s.presLo[1]) ||
+                        (d.sz > 0 ? d.pz > s.presHi[2] : d.pz < s.presLo[2])))
+            break;
     }
     c.cRays++;
     c.cMap += c.sMap;

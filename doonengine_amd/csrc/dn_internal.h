@@ -46,6 +46,10 @@ struct DNsceneView {
     float skyBot[3], skyTop[3];
     const uchar4* cube;          // sky cubemap: 6 faces (+X, -X, +Y, -Y, +Z, -Z) of cubeSize^2 RGBA8, or null
     uint32_t cubeSize;
+    // Box of the present tiles (flags & 3 != 0, loaded or not), inclusive, in
+    // chunk units; lo > hi when there is none. Lighting rays stop once beyond
+    // it on the side they move toward: no voxel, request or stamp lies there.
+    int32_t presLo[3], presHi[3];
 };
 
 // Limits of the per-frame tables passed by value to the lighting kernel.

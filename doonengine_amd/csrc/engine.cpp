@@ -135,6 +135,8 @@ struct DNgpuVolume {
     uint32_t* dRequests = nullptr;
     uint32_t* dNumRequests = nullptr;
     uint32_t* dBlockSums = nullptr;
+    bool presDirty = true;             // presLo/presHi must be rebuilt from state
+    int32_t presLo[3] = {0, 0, 0}, presHi[3] = {0, 0, 0};
     uint32_t* dWorkCounter = nullptr;  // [0] lighting requests taken, [1] collected requests, [2..3] draw quads,
                                        // [4] direct-ray requests taken
     DNdirectTerm* dDirect = nullptr;   // direct-ray terms per request slot (k_direct)
@@ -290,6 +292,7 @@ bool alloc_tile_buffers(DNgpuVolume* g, uint32_t tiles)
         ok = ok && HIP_OK(hipMemsetAsync(b, 0, sizeof(uint32_t) * g->occWords, g->stream), "occupancy clear");
     if(!ok) return false;
     g->state.assign(tiles, 0);
+    g->presDirty = true;
     g->nodeStart.assign(tiles, 0);
     g->nodeClass.assign(tiles, -1);
     g->occLoaded.assign(g->occWords, 0);
@@ -353,6 +356,7 @@ void set_state(DNgpuVolume* g, uint32_t t, uint8_t st)
 {
     uint8_t old = g->state[t];
     if(old == st) return;
+    if((old != 0) != (st != 0)) g->presDirty = true;
     if(old == 1 || old == 3) g->numUnloaded--;
     if(st == 1 || st == 3) g->numUnloaded++;
     g->state[t] = st;
@@ -595,6 +599,25 @@ DNsceneView scene_view(DNvolume* vol)
     s.skyTop[0] = vol->skyGradientTop.x; s.skyTop[1] = vol->skyGradientTop.y; s.skyTop[2] = vol->skyGradientTop.z;
     s.cube = (vol->useCubemap && g->dCube) ? g->dCube : nullptr;
     s.cubeSize = g->cubeSize;
+    if(g->presDirty)
+    {
+        // box of the present tiles, rebuilt after a sync changed which tiles are present
+        int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+        const uint32_t X = vol->mapSize.x, Y = vol->mapSize.y;
+        for(uint32_t t = 0; t < g->tiles; t++)
+            if(g->state[t] != 0)
+            {
+                const int32_t c[3] = {(int32_t)(t % X), (int32_t)((t / X) % Y), (int32_t)(t / (X * Y))};
+                for(int a = 0; a < 3; a++)
+                {
+                    lo[a] = std::min(lo[a], c[a]);
+                    hi[a] = std::max(hi[a], c[a]);
+                }
+            }
+        for(int a = 0; a < 3; a++) { g->presLo[a] = lo[a]; g->presHi[a] = hi[a]; }
+        g->presDirty = false;
+    }
+    for(int a = 0; a < 3; a++) { s.presLo[a] = g->presLo[a]; s.presHi[a] = g->presHi[a]; }
     return s;
 }
 

@@ -164,7 +164,37 @@ typedef struct ORthread {
     uint32_t hitIndex;            /* voxel pool index of the last hit */
     uint32_t sMap, sVox;          /* trips of the last step_map call */
     ORcounters cnt;
+    int stopOutside;              /* lighting rays: stop beyond the box of present tiles (below) */
+    int32_t presLo[3], presHi[3]; /* that box, inclusive; lo > hi when no tile is present */
 } ORthread;
+
+/* The GPU's lighting rays end a walk once it is beyond the box of present
+ * tiles (map flags & 3 != 0) on the side it moves toward: past presHi on an
+ * axis it steps up, below presLo on any other (dn_device.hpp, step_map). No
+ * tile there is entered, requested or stamped, so the walk would end as the
+ * same miss with the same colour terms; only the trip counters differ. The
+ * lighting pass applies the rule while this is set (the default), so the
+ * GPU's counters can be compared; tests clear it to show that the rule
+ * changes nothing else. */
+int or_stop_outside = 1;
+
+static void present_box(const ORscene* s, int32_t lo[3], int32_t hi[3])
+{
+    for(int a = 0; a < 3; a++) { lo[a] = INT32_MAX; hi[a] = INT32_MIN; }
+    for(uint32_t z = 0; z < s->mapSize[2]; z++)
+        for(uint32_t y = 0; y < s->mapSize[1]; y++)
+            for(uint32_t x = 0; x < s->mapSize[0]; x++)
+            {
+                size_t t = (size_t)x + s->mapSize[0] * ((size_t)y + (size_t)s->mapSize[1] * z);
+                if(!(s->map[t * TILE_WORDS] & 3u)) continue;
+                int32_t c[3] = {(int32_t)x, (int32_t)y, (int32_t)z};
+                for(int a = 0; a < 3; a++)
+                {
+                    if(c[a] < lo[a]) lo[a] = c[a];
+                    if(c[a] > hi[a]) hi[a] = c[a];
+                }
+            }
+}
 
 static void thread_init(ORthread* t, int refraction)
 {
@@ -409,6 +439,10 @@ static int step_map(const ORscene* s, ORthread* t, v3* dir, v3 inv, v3* p, int i
         lsd = d.side;
         iterate_dda(&d, hitNormal);
         ignoreFirst = 0;
+        if(t->stopOutside && ((d.step.x > 0 ? d.pos.x > t->presHi[0] : d.pos.x < t->presLo[0]) ||
+                              (d.step.y > 0 ? d.pos.y > t->presHi[1] : d.pos.y < t->presLo[1]) ||
+                              (d.step.z > 0 ? d.pos.z > t->presHi[2] : d.pos.z < t->presLo[2])))
+            break;
     }
     t->cnt.calls++;
     t->cnt.sMap += t->sMap;
@@ -913,6 +947,8 @@ static void light_range(LRange* j)
     sc.lastUsedOut = j->lastW;
     ORthread t;
     thread_init(&t, 0);
+    t.stopOutside = or_stop_outside;
+    present_box(s, t.presLo, t.presHi);
     LCtx c;
     c.s = &sc; c.lp = lp; c.t = &t; c.visProp = j->prop;
     c.sunDir = vf(lp->sunDir);

@@ -5,6 +5,7 @@ restatement (a dropped term, a wrong index, a swapped operand) fails here.
 """
 from __future__ import annotations
 
+import ctypes
 import json
 import math
 from pathlib import Path
@@ -610,3 +611,42 @@ def test_threaded_oracle_equals_single_thread(o, o64):
                 assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
             assert {k: got[5][k] for k in ("calls", "sMap", "sVox", "hits")} == \
                 {k: base[5][k] for k in ("calls", "sMap", "sVox", "hits")}
+
+
+def test_stop_outside_present_box_changes_only_counters(o, o64):
+    """Lighting rays end once they are beyond the box of present tiles on the
+    side they move toward (the rule of the GPU's lighting walk, which the
+    oracle applies while or_stop_outside is set). With the rule on and off the
+    lighting pass gives the same stored words, pre-quantisation values, flags
+    (visible bits and the requests of unloaded tiles), lastUsed stamps and
+    sample counts, and the same rays, voxel trips and hits; only the
+    chunk-grid trips drop."""
+    sc = scenes.config_small(seed=17, chunks=(6, 5, 6), occupancy=0.2)
+    for k in range(4):
+        sc.materials[k].specular = 0.5 if k == 1 else 0.0
+        sc.materials[k].shininess = 4
+    m, c, v, mats = build_state(sc)
+    mats[1]["specular"], mats[1]["shininess"] = 0.5, 4
+    loaded = np.nonzero((m[:, 0] & 3) == 2)[0]
+    req, mm = _all_requests(o, m, c)
+    mm[loaded[::5], 0] = (mm[loaded[::5], 0] & ~np.uint32(7)) | 1     # present, not loaded: rays request them
+    req = o.visible_requests(mm, c)
+    assert len(req) > 20
+    kw = dict(time=321.0, sun_dir=np.array([-0.57735, 0.57735, -0.57735], np.float32), num_diffuse=2,
+              spec_points=oracle.golden_points(15).astype(np.float32), direct_points=oracle.golden_points(8),
+              cam_pos=sc.cam_pos)
+    for orc in (o, o64):
+        flag = ctypes.c_int.in_dll(orc.lib, "or_stop_outside")
+        s = orc.scene(sc.map_size, mm, c, v, mats)
+        try:
+            flag.value = 0
+            full = orc.lighting(s, req, **kw)
+            flag.value = 1
+            stop = orc.lighting(s, req, **kw)
+        finally:
+            flag.value = 1
+        for a, b in zip(full[:5], stop[:5]):
+            assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
+        assert ((stop[1][:, 0] & 3) == 3).any()                         # some requests were made
+        assert {k: stop[5][k] for k in ("calls", "sVox", "hits")} == {k: full[5][k] for k in ("calls", "sVox", "hits")}
+        assert stop[5]["sMap"] < 0.9 * full[5]["sMap"]
