@@ -365,3 +365,36 @@ def test_draw_parity_odd_image_sizes(gpu, size):
     cnt = vol.counters(0, reset=True)
     assert cnt["rays"] == ocnt["calls"] == w * h
     vol.close()
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3, 4, 5])
+def test_draw_view_modes_parity(gpu, mode):
+    """DN_draw's other view modes (voxelDraw.comp:32-61: albedo, diffuse light,
+    specular light, voxel normal, face normal) on a lit scene with a glossy
+    material, against the oracle's draw in the same mode."""
+    sc = scenes.config_small(seed=14)
+    sc.materials[1].specular = 0.5
+    sc.materials[1].shininess = 8
+    o = oracle.load("f32")
+    vol = load_scene(sc)
+    vol.v.numDirectRays = 4
+    for f in range(2):                               # stored diffuse and specular light that is not zero
+        rgba0, _, _, _ = draw_gpu(vol, sc)
+        vol.mark_visible(1 << 20)
+        vol.sync()
+        vol.update_lighting(1, 1000, 5.0 + f)
+    vol.synchronize()
+    vol.v.camViewMode = mode
+    s, (m, _, v) = oracle_scene(o, vol, sc)
+    assert (v[:, 3] != 0).any()
+    rgba, px, view, proj = draw_gpu(vol, sc)
+    from doonengine_amd.volume import draw_matrices
+    iv, ivp = draw_matrices(view, proj)
+    orgba, opx, _ = o.draw(s, mat_to_np(iv), mat_to_np(ivp), sc.width, sc.height, view_mode=mode,
+                           flags_out=m[:, 0].copy())
+    assert (px["voxelIndex"] != 0xFFFFFFFF).mean() > 0.2
+    assert np.array_equal(px["voxelIndex"], opx["voxelIndex"])
+    assert np.array_equal(rgba[..., 3], orgba[..., 3])
+    np.testing.assert_allclose(rgba[..., :3], orgba[..., :3], rtol=2e-6, atol=1e-7)
+    assert not np.allclose(rgba[..., :3], rgba0[..., :3])      # the mode changes the image
+    vol.close()
