@@ -110,30 +110,48 @@ struct Corners {
     float w[8];
 };
 
-__device__ __forceinline__ void level_corners(const DNrcNet& net, uint32_t l, const float* pos, Corners& c)
-{
-    const uint32_t N = net.res[l];
-    const uint32_t T = 1u << net.log2T;
-    float f[3];
+// The cell of a level that holds a position: its lower corner g and the
+// position's fractions f inside it.
+struct LevelCell {
     uint32_t g[3];
+    float f[3];
+};
+
+__device__ __forceinline__ LevelCell level_cell(const DNrcNet& net, uint32_t l, const float* pos)
+{
+    LevelCell c;
+    const uint32_t N = net.res[l];
     for(int a = 0; a < 3; a++)
     {
         float x = pos[a] / net.extent;
         x = fminf(fmaxf(x, 0.0f), 1.0f);
         float s = x * (float)N + 0.5f;
         float fl = floorf(s);
-        g[a] = (uint32_t)fl;
-        f[a] = s - fl;
+        c.g[a] = (uint32_t)fl;
+        c.f[a] = s - fl;
     }
+    return c;
+}
+
+// Corner k (bit 0: x + 1, bit 1: y + 1, bit 2: z + 1) of a cell: its table
+// entry (dense index, or the spatial hash) and its trilinear weight.
+__device__ __forceinline__ void cell_corner(const DNrcNet& net, uint32_t l, const LevelCell& c, uint32_t k,
+                                            uint32_t& idx, float& w)
+{
+    const uint32_t T = 1u << net.log2T;
     const bool dense = (net.denseMask >> l) & 1u;
-    const uint32_t S = N + 2u;
-    for(uint32_t k = 0; k < 8; k++)
-    {
-        uint32_t cx = g[0] + (k & 1u), cy = g[1] + ((k >> 1) & 1u), cz = g[2] + ((k >> 2) & 1u);
-        uint32_t idx = dense ? cx + cy * S + cz * S * S : (cx * 1u) ^ (cy * 2654435761u) ^ (cz * 805459861u);
-        c.idx[k] = idx & (T - 1u);
-        c.w[k] = ((k & 1u) ? f[0] : 1.0f - f[0]) * ((k & 2u) ? f[1] : 1.0f - f[1]) * ((k & 4u) ? f[2] : 1.0f - f[2]);
-    }
+    const uint32_t S = net.res[l] + 2u;
+    const uint32_t cx = c.g[0] + (k & 1u), cy = c.g[1] + ((k >> 1) & 1u), cz = c.g[2] + ((k >> 2) & 1u);
+    const uint32_t i = dense ? cx + cy * S + cz * S * S : (cx * 1u) ^ (cy * 2654435761u) ^ (cz * 805459861u);
+    idx = i & (T - 1u);
+    w = ((k & 1u) ? c.f[0] : 1.0f - c.f[0]) * ((k & 2u) ? c.f[1] : 1.0f - c.f[1]) *
+        ((k & 4u) ? c.f[2] : 1.0f - c.f[2]);
+}
+
+__device__ __forceinline__ void level_corners(const DNrcNet& net, uint32_t l, const float* pos, Corners& c)
+{
+    const LevelCell cell = level_cell(net, l, pos);
+    for(uint32_t k = 0; k < 8; k++) cell_corner(net, l, cell, k, c.idx[k], c.w[k]);
 }
 
 // One thread per (sample, level): the level's two features into x0, level 0
@@ -164,77 +182,41 @@ __global__ __launch_bounds__(256) void k_rc_encode(DNrcNet net, const float* sam
     }
 }
 
-// Levels whose table gradients a wave sums per distinct entry before its
-// atomics. Training samples come in runs of 32 voxels of one lighting request
-// (one chunk), so at the coarse levels a wave's 64 samples share a few cells.
-#ifndef RC_AGG_LEVELS
-#define RC_AGG_LEVELS 8u
-#endif
-
-// gt[2 key] += v0 and gt[2 key + 1] += v1 for every active lane, with one
-// atomic pair per distinct key of the wave: a leader lane's key is matched by
-// every lane holding it, their values are summed across the wave, and the
-// leader adds the sums. Each round retires at least the leader.
-__device__ __forceinline__ void wave_add_pairs(float* gt, uint32_t key, float v0, float v1, bool active)
-{
-    const uint32_t lane = threadIdx.x & 63u;
-    uint64_t pending = __ballot(active);
-    while(pending)
-    {
-        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)pending) - 1u;
-        const uint32_t lk = (uint32_t)__shfl((int)key, (int)leader, 64);
-        const bool mine = active && key == lk;
-        float s0 = mine ? v0 : 0.0f, s1 = mine ? v1 : 0.0f;
-        for(int o = 32; o > 0; o >>= 1)
-        {
-            s0 += __shfl_xor(s0, o, 64);
-            s1 += __shfl_xor(s1, o, 64);
-        }
-        if(lane == leader)
-        {
-            atomicAdd(gt + 2u * lk, s0);
-            atomicAdd(gt + 2u * lk + 1u, s1);
-        }
-        pending &= ~__ballot(mine);
-        active = active && !mine;
-    }
-}
-
-// Hash-table gradient: each wave takes 64 consecutive samples at one level
-// and spreads the level's two feature gradients over the 8 corners by their
-// weights. The FP32 atomics bound it: without the per-wave sums of the 8
-// coarsest levels it takes 1224 us per 2^17 samples, with them at 4 levels 961,
-// 6 levels 870, 8 levels 820.
+// Hash-table gradient, level-major: the waves running at a time work on one
+// level's gradient table (4 MiB at 2^19 entries), finest level first.
+// Device-scope FP32 atomics are served per cache line that one instruction
+// touches, not per lane (measured on MI355X: 64 lanes on 64 scattered lines
+// sustain 20.7 G atomics/s, lane pairs on one 8-byte entry 41 G/s, 16 lanes on
+// one 64-byte run about 290 G/s). So four lanes take one sample: lane
+// q = 2 xb + f adds feature f at the four corners whose x offset is xb. A
+// sample's two x-neighbour corners are adjacent entries on a dense level
+// (x + S (y + S z)) and, on a hashed level (x ^ h(y, z)), lie in one aligned
+// run of entries unless x + 1 carries past it, so each instruction sends two
+// corners and both features of a sample to one line. Per 2^17 training
+// samples this takes 185 us, against 378 us with one lane per sample (two
+// instructions per corner) and per-wave sums of shared entries on the 8
+// coarsest levels; with the four-lane layout such sums only cost time
+// (8 levels 275 us, 6 levels 239, 4 levels 223).
 __global__ __launch_bounds__(256) void k_rc_table_grad(DNrcNet net, const float* samples, uint32_t n, const float* denc)
 {
     const uint32_t g = blockIdx.x * 256u + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u, w = g >> 6;
-    // level-major: the waves running at a time work on one level's gradient
-    // table (4 MiB at 2^19 entries) instead of all sixteen (820 against 840 us
-    // per 2^17 samples); finest level first. Waves past the last level idle.
-    const uint32_t sbs = (n + 63u) / 64u, lvl = w / sbs;
-    const uint32_t l = 15u - min(lvl, 15u), s = lvl < 16u ? (w % sbs) * 64u + lane : n;
-    float d0 = 0.0f, d1 = 0.0f;
-    if(s < n)
-    {
-        d0 = denc[(size_t)s * RC_ENC + 2 * l];
-        d1 = denc[(size_t)s * RC_ENC + 2 * l + 1];
-    }
-    const bool active = d0 != 0.0f || d1 != 0.0f;
+    const uint32_t waves = (n + 15u) / 16u;                      // per level
+    if(w >= RC_LEVELS * waves) return;
+    const uint32_t l = RC_LEVELS - 1u - w / waves;
+    const uint32_t s = (w % waves) * 16u + (lane >> 2);
+    const uint32_t f = lane & 1u, xb = (lane >> 1) & 1u;
+    if(s >= n) return;
+    const float d = denc[(size_t)s * RC_ENC + 2u * l + f];
+    if(d == 0.0f) return;                                       // adding +-0 changes no sum
     float* gt = net.tabGrad + ((size_t)l << net.log2T) * 2u;
-    Corners c;
-    level_corners(net, l, samples + (size_t)(s < n ? s : 0u) * RC_SAMPLE, c);
-    if(l < RC_AGG_LEVELS)
+    const LevelCell cell = level_cell(net, l, samples + (size_t)s * RC_SAMPLE);
+    for(uint32_t j = 0; j < 4u; j++)
     {
-        for(int k = 0; k < 8; k++) wave_add_pairs(gt, c.idx[k], c.w[k] * d0, c.w[k] * d1, active);
-    }
-    else if(active)
-    {
-        for(int k = 0; k < 8; k++)
-        {
-            atomicAdd(gt + 2u * c.idx[k], c.w[k] * d0);
-            atomicAdd(gt + 2u * c.idx[k] + 1u, c.w[k] * d1);
-        }
+        uint32_t idx;
+        float wt;
+        cell_corner(net, l, cell, (j << 1) | xb, idx, wt);
+        atomicAdd(gt + 2u * idx + f, wt * d);
     }
 }
 
@@ -734,9 +716,9 @@ int dn_rc_train(const DNrcNet& net, const float* samples, uint32_t n, const DNrc
     hipLaunchKernelGGL(k_rc_encode, dim3(blocks_for((size_t)n * 16u)), dim3(256), 0, stream, net, samples, n, a.x0);
     hipLaunchKernelGGL(k_rc_mlp<true>, dim3(blocks_for((size_t)tiles * 64u, 1024)), dim3(256), 0, stream, net,
                        (const uint16_t*)a.x0, samples, n, 1.0f / (float)n, (float*)nullptr, a);
-    // 16 waves (one per level) per 64 samples
-    hipLaunchKernelGGL(k_rc_table_grad, dim3(blocks_for((size_t)((n + 63u) / 64u) * 1024u)), dim3(256), 0, stream, net,
-                       samples, n, (const float*)a.denc);
+    // one wave per 16 samples and level
+    hipLaunchKernelGGL(k_rc_table_grad, dim3(blocks_for((size_t)RC_LEVELS * ((n + 15u) / 16u) * 64u)), dim3(256), 0,
+                       stream, net, samples, n, (const float*)a.denc);
     uint32_t wb = tiles < 128u ? tiles : 128u;
     hipLaunchKernelGGL(k_rc_wgrad, dim3(wb), dim3(256), 0, stream, net, tiles, a);
     return (int)hipGetLastError();
