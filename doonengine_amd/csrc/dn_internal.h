@@ -241,8 +241,9 @@ bool dn_noalloc_init();
 int dn_rc_predict_noalloc(const DNrcNet& net, int device, const float* samples, uint32_t n, float* pred,
                           hipStream_t stream);
 int dn_rc_adam(const DNrcNet& net, const DNrcAdam& opt, hipStream_t stream);
-int dn_rc_samples(const DNlightParams& p, uint32_t maxRequests, uint32_t n, uint32_t seed, float* samples,
-                  hipStream_t stream);
+// `batches` batches of n training samples, batch k drawn with seed + k * seedStride
+int dn_rc_samples(const DNlightParams& p, uint32_t maxRequests, uint32_t n, uint32_t batches, uint32_t seed,
+                  uint32_t seedStride, float* samples, hipStream_t stream);
 int dn_rc_gather(const DNlightParams& p, uint32_t slots, float* samples, hipStream_t stream);
 int dn_accumulate_f64(const float* src, double* dst, size_t n, hipStream_t stream);
 // acc[slot] (4 doubles: rgb sum, count) += samplesPerVoxel (rounded up to the

@@ -19,7 +19,9 @@ struct DNXradianceCache {
     DNrcNet net;
     DNrcActs acts;
     uint32_t cap = 0;             // rows of the training buffers
-    float* dSamples = nullptr;    // cap rows
+    float* dSamples = nullptr;    // sampleCap rows: the drawn training samples of up to 16 steps
+    size_t sampleCap = 0;
+    const float* lastSamplesPtr = nullptr;   // the last step's rows in dSamples
     uint32_t lastSamples = 0;
     float* dQuery = nullptr;      // query rows for predictions
     uint16_t* dQueryX0 = nullptr;
@@ -104,14 +106,25 @@ bool ensure_train(DNXradianceCache* rc, uint32_t n)
     if(n <= rc->cap) return true;
     DNrcActs& a = rc->acts;
     hipStreamSynchronize(rc->stream);
-    dfree(rc->dSamples); dfree(a.x0); dfree(a.x0T); dfree(a.dyT); dfree(a.denc);
+    dfree(a.x0); dfree(a.x0T); dfree(a.dyT); dfree(a.denc);
     for(int i = 0; i < 3; i++) { dfree(a.hT[i]); dfree(a.dzT[i]); }
-    bool g = dev(&rc->dSamples, (size_t)n * RC_SAMPLE, "samples") && dev(&a.x0, (size_t)n * RC_IN, "activations") &&
-             dev(&a.x0T, (size_t)n * RC_IN, "activations") && dev(&a.dyT, (size_t)n * RC_OUT, "activations") &&
-             dev(&a.denc, (size_t)n * RC_ENC, "activations");
+    bool g = dev(&a.x0, (size_t)n * RC_IN, "activations") && dev(&a.x0T, (size_t)n * RC_IN, "activations") &&
+             dev(&a.dyT, (size_t)n * RC_OUT, "activations") && dev(&a.denc, (size_t)n * RC_ENC, "activations");
     for(int i = 0; g && i < 3; i++)
         g = dev(&a.hT[i], (size_t)n * RC_W, "activations") && dev(&a.dzT[i], (size_t)n * RC_W, "activations");
     rc->cap = g ? n : 0;
+    return g;
+}
+
+bool ensure_samples(DNXradianceCache* rc, size_t rows)
+{
+    if(rows <= rc->sampleCap) return true;
+    hipStreamSynchronize(rc->stream);
+    dfree(rc->dSamples);
+    rc->lastSamplesPtr = nullptr;
+    rc->lastSamples = 0;
+    const bool g = dev(&rc->dSamples, rows * RC_SAMPLE, "samples");
+    rc->sampleCap = g ? rows : 0;
     return g;
 }
 
@@ -333,15 +346,27 @@ extern "C" bool DNX_rc_train_frame(DNXradianceCache* rc, float time, int steps)
     if(!prepare_lighting(rc->vol, 1, 1000, time, p, maxReq)) return false;
     const uint32_t n = rc->cfg.samplesPerFrame;
     if(!ensure_train(rc, n)) return false;
-    // every step on samples of its own (a step's draws depend on the frame
-    // counter, which advances per step)
-    for(int s = 0; s < steps; s++)
+    if(steps <= 0) return true;
+    // Every step trains on samples of its own, drawn with the seed of its
+    // frame counter, which advances per step: seed = base + frame * stride.
+    // The samples of up to 16 steps are drawn in one launch (the draws of one
+    // step, 2^17 lanes, fill two waves per SIMD; the training kernels of a step
+    // do not read the volume, so drawing ahead changes nothing).
+    const uint32_t group = steps < 16 ? (uint32_t)steps : 16u;
+    if(!ensure_samples(rc, (size_t)group * n)) return false;
+    for(int s0 = 0; s0 < steps; s0 += (int)group)
     {
+        const uint32_t m = (uint32_t)(steps - s0) < group ? (uint32_t)(steps - s0) : group;
         const uint32_t seed = rc->cfg.seed * 0x9E3779B1u + rc->frame * 0x85EBCA77u + 0x632BE5ABu;
-        rc->frame++;
-        if(dn_rc_samples(p, maxReq, n, seed, rc->dSamples, rc->stream) != 0) return false;
-        rc->lastSamples = n;
-        if(dn_rc_train(rc->net, rc->dSamples, n, rc->acts, rc->stream) != 0 || !adam_step(rc)) return false;
+        if(dn_rc_samples(p, maxReq, n, m, seed, 0x85EBCA77u, rc->dSamples, rc->stream) != 0) return false;
+        for(uint32_t k = 0; k < m; k++)
+        {
+            rc->frame++;
+            const float* smp = rc->dSamples + (size_t)k * n * RC_SAMPLE;
+            rc->lastSamplesPtr = smp;
+            rc->lastSamples = n;
+            if(dn_rc_train(rc->net, smp, n, rc->acts, rc->stream) != 0 || !adam_step(rc)) return false;
+        }
     }
     return true;
 }
@@ -380,7 +405,7 @@ extern "C" const float* DNX_rc_last_samples(DNXradianceCache* rc, uint32_t* n)
 {
     if(!rc) return nullptr;
     if(n) *n = rc->lastSamples;
-    return rc->dSamples;
+    return rc->lastSamplesPtr;
 }
 
 extern "C" bool DNX_accumulate_f64(DNvolume* vol, const float* src, double* dst, size_t n)

@@ -542,11 +542,16 @@ __device__ __forceinline__ void site_ctx(const DNlightParams& P, const DNsceneVi
     L.direct = nullptr;
 }
 
-__global__ __launch_bounds__(256) void k_rc_samples(DNlightParams P, uint32_t n, uint32_t seed, float* samples)
+// Lane G draws sample g = G mod n of batch k = G / n, with that batch's seed
+// seed0 + k * seedStride (the batches of several training steps in one launch).
+__global__ __launch_bounds__(256) void k_rc_samples(DNlightParams P, uint32_t n, uint32_t total, uint32_t seed0,
+                                                   uint32_t seedStride, float* samples)
 {
     __shared__ uint32_t maskRows[DN_MASK_LDS_WORDS(256)];
-    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
-    if(g >= n) return;
+    const uint32_t G = blockIdx.x * 256u + threadIdx.x;
+    if(G >= total) return;
+    const uint32_t k = G / n, g = G - k * n;
+    const uint32_t seed = seed0 + k * seedStride;
     const uint32_t slots = *P.numRequests * 32u;
     const uint32_t base = pcg(seed ^ pcg(g));
     const DNchunkRec* rec = nullptr;
@@ -562,7 +567,7 @@ __global__ __launch_bounds__(256) void k_rc_samples(DNlightParams P, uint32_t n,
                                      : pcg(base + t * 0x68E31DA4u) % slots;
         found = slot_voxel(P, slot, rec, mapIndex, voxNum, sel);
     }
-    float* out = samples + (size_t)g * RC_SAMPLE;
+    float* out = samples + (size_t)G * RC_SAMPLE;
     if(!found)
     {
         write_sample(out, splat(0.0f), DVoxel{}, splat(0.0f), 0.0f);
@@ -732,12 +737,15 @@ int dn_rc_adam(const DNrcNet& net, const DNrcAdam& opt, hipStream_t stream)
     return (int)hipGetLastError();
 }
 
-int dn_rc_samples(const DNlightParams& p, uint32_t maxRequests, uint32_t n, uint32_t seed, float* samples,
-                  hipStream_t stream)
+int dn_rc_samples(const DNlightParams& p, uint32_t maxRequests, uint32_t n, uint32_t batches, uint32_t seed,
+                  uint32_t seedStride, float* samples, hipStream_t stream)
 {
     (void)maxRequests;
-    if(n == 0) return 0;
-    hipLaunchKernelGGL(k_rc_samples, dim3(blocks_for(n)), dim3(256), 0, stream, p, n, seed, samples);
+    if(n == 0 || batches == 0) return 0;
+    if((uint64_t)n * batches > 0xFFFFFFFFull) return (int)hipErrorInvalidValue;
+    const uint32_t total = n * batches;
+    hipLaunchKernelGGL(k_rc_samples, dim3(blocks_for(total)), dim3(256), 0, stream, p, n, total, seed, seedStride,
+                       samples);
     return (int)hipGetLastError();
 }
 

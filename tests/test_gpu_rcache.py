@@ -97,6 +97,34 @@ def test_gradients_match_oracle(small_volume):
         rc.close()
 
 
+def test_steps_drawn_together_equal_steps_drawn_alone(small_volume):
+    """DNX_rc_train_frame draws the training samples of several steps in one
+    launch, batch k with the seed of the frame counter plus k: the last step's
+    samples of a 3-step frame equal, bit for bit, those of the third of three
+    1-step frames, and differ from the first step's."""
+    vol, _ = small_volume
+    vol.mark_visible(1 << 20)
+    vol.sync()
+    vol.synchronize()
+    a = RadianceCache(vol, log2Table=LOG2T, extent=128.0, samplesPerFrame=4096, seed=5)
+    b = RadianceCache(vol, log2Table=LOG2T, extent=128.0, samplesPerFrame=4096, seed=5)
+    try:
+        a.train_frame(10.0, 3)
+        sa = a.last_samples()
+        b.train_frame(10.0, 1)
+        first = b.last_samples()
+        b.train_frame(10.0, 1)
+        b.train_frame(10.0, 1)
+        sb = b.last_samples()
+        assert sa.shape == (4096, 16)
+        assert (sa[:, 12] == 1.0).sum() > 2000                       # rows holding a drawn voxel
+        assert np.array_equal(sa.view(np.uint32), sb.view(np.uint32))
+        assert not np.array_equal(sa.view(np.uint32), first.view(np.uint32))
+    finally:
+        a.close()
+        b.close()
+
+
 def test_adam_first_step(small_volume):
     vol, _ = small_volume
     cfg = orc.Config(log2_table=LOG2T, extent=128.0)
