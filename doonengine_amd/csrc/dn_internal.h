@@ -189,6 +189,20 @@ enum {
     RC_LEVELS = 16, RC_FEAT = 2, RC_ENC = 32, RC_IN = 48, RC_W = 64, RC_OUT = 16, RC_SAMPLE = 16,
     RC_OFF_W0 = 0, RC_OFF_W1 = 3072, RC_OFF_W2 = 7168, RC_OFF_W3 = 11264, RC_NPARAM = 12288
 };
+
+// Index in the transposed weight layout (each of W0 64x48, W1 and W2 64x64,
+// W3 16x64 transposed in place, same offsets) of weight i of the row-major one.
+__host__ __device__ inline uint32_t rc_transposed_index(uint32_t i)
+{
+    uint32_t off, rows, cols;
+    if(i < RC_OFF_W1) { off = RC_OFF_W0; rows = 64; cols = 48; }
+    else if(i < RC_OFF_W2) { off = RC_OFF_W1; rows = 64; cols = 64; }
+    else if(i < RC_OFF_W3) { off = RC_OFF_W2; rows = 64; cols = 64; }
+    else { off = RC_OFF_W3; rows = 16; cols = 64; }
+    const uint32_t j = i - off;
+    return off + (j % cols) * rows + j / cols;
+}
+
 struct DNrcNet {
     uint32_t log2T;
     uint32_t denseMask;            // bit l: level l indexes densely ((N_l + 2)^3 <= T)
@@ -200,6 +214,7 @@ struct DNrcNet {
     float* tabM;
     float* tabV;
     uint16_t* w16;                 // [RC_NPARAM] bf16
+    uint16_t* wT16;                // w16 in the transposed layout (rc_transposed_index), for backward
     float* w32;                    // master copy
     float* wGrad;
     float* wM;

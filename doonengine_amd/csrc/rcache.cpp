@@ -80,13 +80,15 @@ uint16_t host_bf16(float f)
 bool upload_params(DNXradianceCache* rc, const float* w, const float* tab)
 {
     const size_t tf = table_floats(rc);
-    std::vector<uint16_t> w16(RC_NPARAM);
+    std::vector<uint16_t> w16(RC_NPARAM), wT16(RC_NPARAM);
     for(int i = 0; i < RC_NPARAM; i++) w16[i] = host_bf16(w[i]);
+    for(uint32_t i = 0; i < RC_NPARAM; i++) wT16[rc_transposed_index(i)] = w16[i];
     std::vector<uint32_t> t16(tf / 2);
     for(size_t i = 0; i < tf / 2; i++) t16[i] = host_bf16(tab[2 * i]) | ((uint32_t)host_bf16(tab[2 * i + 1]) << 16);
     DNrcNet& n = rc->net;
     bool good = ok(hipMemcpyAsync(n.w32, w, sizeof(float) * RC_NPARAM, hipMemcpyHostToDevice, rc->stream), "upload") &&
                 ok(hipMemcpyAsync(n.w16, w16.data(), 2 * RC_NPARAM, hipMemcpyHostToDevice, rc->stream), "upload") &&
+                ok(hipMemcpyAsync(n.wT16, wT16.data(), 2 * RC_NPARAM, hipMemcpyHostToDevice, rc->stream), "upload") &&
                 ok(hipMemcpyAsync(n.tab32, tab, sizeof(float) * tf, hipMemcpyHostToDevice, rc->stream), "upload") &&
                 ok(hipMemcpyAsync(n.tab16, t16.data(), 4 * (tf / 2), hipMemcpyHostToDevice, rc->stream), "upload");
     good = good && ok(hipMemcpyAsync(n.wE16, w16.data(), 2 * RC_NPARAM, hipMemcpyHostToDevice, rc->stream), "upload") &&
@@ -200,7 +202,7 @@ extern "C" void DNX_rc_destroy(DNXradianceCache* rc)
     if(rc->stream) hipStreamSynchronize(rc->stream);
     DNrcNet& n = rc->net;
     dfree(n.tab16); dfree(n.tab32); dfree(n.tabGrad); dfree(n.tabM); dfree(n.tabV); dfree(n.tabE); dfree(n.tabE16);
-    dfree(n.w16); dfree(n.w32); dfree(n.wGrad); dfree(n.wM); dfree(n.wV); dfree(n.wE); dfree(n.wE16);
+    dfree(n.w16); dfree(n.wT16); dfree(n.w32); dfree(n.wGrad); dfree(n.wM); dfree(n.wV); dfree(n.wE); dfree(n.wE16);
     DNrcActs& a = rc->acts;
     dfree(rc->dSamples); dfree(a.x0); dfree(a.x0T); dfree(a.dyT); dfree(a.denc);
     for(int i = 0; i < 3; i++) { dfree(a.hT[i]); dfree(a.dzT[i]); }
@@ -250,7 +252,7 @@ extern "C" DNXradianceCache* DNX_rc_create(DNvolume* vol, const DNXrcConfig* con
     const size_t tf = table_floats(rc);
     bool g = dev(&n.tab16, tf / 2, "hash table") && dev(&n.tab32, tf, "hash table") && dev(&n.tabGrad, tf, "hash table") &&
              dev(&n.tabM, tf, "hash table") && dev(&n.tabV, tf, "hash table") && dev(&n.tabE, tf, "hash table") &&
-             dev(&n.tabE16, tf / 2, "hash table") && dev(&n.w16, RC_NPARAM, "weights") &&
+             dev(&n.tabE16, tf / 2, "hash table") && dev(&n.w16, RC_NPARAM, "weights") && dev(&n.wT16, RC_NPARAM, "weights") &&
              dev(&n.w32, RC_NPARAM, "weights") && dev(&n.wGrad, RC_NPARAM, "weights") &&
              dev(&n.wM, RC_NPARAM, "weights") && dev(&n.wV, RC_NPARAM, "weights") && dev(&n.wE, RC_NPARAM, "weights") &&
              dev(&n.wE16, RC_NPARAM, "weights");
