@@ -49,7 +49,7 @@ FRAME_SOURCES = ("dn_device.hpp", "dn_internal.h", "dn_light.hpp", "engine.cpp",
 def source_stamp() -> str:
     """First 16 hex digits of the SHA-256 of the sources the frame's kernels
     and their launches are built from (doonengine_amd/csrc), in name order:
-    the build a profile was taken of."""
+    which sources a profile was taken of."""
     h = hashlib.sha256()
     for name in FRAME_SOURCES:
         f = ROOT / "doonengine_amd" / "csrc" / name
@@ -508,7 +508,7 @@ def main() -> int:
                          "size, to scale profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
         return 2
     if args.workload == "batch":
@@ -676,8 +676,8 @@ def main() -> int:
 
 def roofline(light_bytes: float, light_ms: float, achieved: float) -> dict:
     """The lighting pass against the HBM roofline on the traversal model's
-    algorithmic bytes, with what the committed profile of this build measured
-    beside it: the real HBM traffic (so the fraction of HBM bandwidth actually
+    algorithmic bytes, with what the committed profile of the same kernel
+    sources measured beside it: the real HBM traffic (so the fraction of HBM bandwidth actually
     used) and the VALU counters that show what bounds the kernels."""
     pv = profile_values()
     traffic = fnum(pv, "traffic_bytes_per_launch lighting_pass")

@@ -268,8 +268,8 @@ __device__ __forceinline__ uint32_t rank_row(const uint32_t* row, const uint32_t
     return n + __builtin_popcount(b0) + __builtin_popcount(b1) + __builtin_popcount(b2) + __builtin_popcount(b3);
 }
 
-// step_chunk, voxelShared.comp:328-418. maxDepth is negative on every path of
-// this build (no raster compositing), so the depth test at :363 always passes.
+// step_chunk, voxelShared.comp:328-418. maxDepth is negative on every path here
+// (no raster compositing), so the depth test at :363 always passes.
 // LDSM: the chunk's occupancy mask goes to the lane's LDS row (c.maskRow) on
 // entry and each voxel trip reads one word of it, instead of selecting the
 // word out of 16 registers by a per-lane index. FINITE: the ray's delta is

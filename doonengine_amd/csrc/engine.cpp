@@ -1,10 +1,11 @@
 // Host engine: the reference's C API (src/DoonEngine/voxel.h:156-371) over HIP
 // device state. CPU-side chunk storage and the edit API keep the reference's
-// semantics (voxel.c:659-714, 1101-1316); the GPU side is this build's own
-// design: per-tile SoA device arrays, a buddy allocator for the voxel pool,
-// one staged upload per sync, and device-side visible-chunk compaction.
+// semantics (voxel.c:659-714, 1101-1316); the GPU side is designed for HIP:
+// per-tile SoA device arrays, a buddy allocator for the voxel pool, one staged
+// upload per sync, and device-side visible-chunk compaction.
 #include "doon/doon.h"
 #include "dn_internal.h"
+#include "rc_internal.h"
 
 #include <algorithm>
 #include <atomic>

@@ -2,7 +2,7 @@
 // parameter storage, initialisation, the per-frame train / predict sequence
 // and the C-ABI entry points declared in doon/doon.h.
 #include "doon/doon.h"
-#include "dn_internal.h"
+#include "rc_internal.h"
 
 #include <algorithm>
 #include <cmath>

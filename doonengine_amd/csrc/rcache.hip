@@ -16,6 +16,7 @@
 // split-K MFMA reduction over the tiles, hash-table gradients by FP32 atomics,
 // then Adam (Kingma & Ba 2015, algorithm 1) on FP32 master copies.
 #include "dn_light.hpp"
+#include "rc_internal.h"
 #include "doon/doon.h"
 
 #include <mutex>

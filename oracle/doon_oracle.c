@@ -1113,7 +1113,7 @@ void or_lighting(const ORscene* s, const ORlightParams* lp, const uint32_t* requ
 
 /* ------------------------------------------------------------------------- */
 /* chunk conversion (voxel.c:1391-1461). Host C code in the reference: always  */
-/* FP32, whatever this build's real type is.                                   */
+/* FP32, whatever the oracle's `real` type is.                                 */
 
 /* cpuVoxels: DNchunk.voxels[8][8][8] of {normal, albedo}, flat index ((x*8)+y)*8+z.
  * Writes chunkOut (24 words; pos and sample count left to the caller) and up to
