@@ -409,9 +409,14 @@ __device__ bool step_map(const DNsceneView& s, RayCtx& c, f3& dir, f3 inv, f3& p
     LastSide<FINITE> lsd;
     bool hit = false;
     const int32_t X = (int32_t)s.mapX, Y = (int32_t)s.mapY, Z = (int32_t)s.mapZ;
+    // a lighting ray that does not move down reads the column table of its
+    // quadrant: the highest present tile over the columns it can still reach
+    const bool colRule = !REFRACT && s.colTop != nullptr && d.sy >= 0;
+    const uint32_t colBase = ((d.sx > 0 ? 1u : 0u) + (d.sz > 0 ? 2u : 0u)) * s.mapX * s.mapZ;
     while(in_map_cell(d, (uint32_t)X, (uint32_t)Y, (uint32_t)Z))
     {
         c.sMap++;
+        const int32_t colTop = colRule ? s.colTop[colBase + (uint32_t)d.px + s.mapX * (uint32_t)d.pz] : INT32_MAX;
         uint32_t idx = (uint32_t)d.px + s.mapX * ((uint32_t)d.py + s.mapY * (uint32_t)d.pz);
         if(occ_bit(s.occLoaded, idx))
         {
@@ -447,11 +452,14 @@ __device__ bool step_map(const DNsceneView& s, RayCtx& c, f3& dir, f3 inv, f3& p
         // toward (a positive step past presHi, any other step below presLo)
         // meets no loaded or unloaded tile before it leaves the map, so the
         // rest of the walk would only count trips: it ends here as the same
-        // miss, with the same colour terms. Only the trip counters differ from
-        // the reference walk's; the oracle applies the same rule.
+        // miss, with the same colour terms. So does a ray that does not move
+        // down once it is above every present tile of the columns it could
+        // still reach from the cell it just left (its y only grows and those
+        // columns include all of its later ones). Only the trip counters
+        // differ from the reference walk's; the oracle applies the same rules.
         if(!REFRACT && ((d.sx > 0 ? d.px > s.presHi[0] : d.px < s.presLo[0]) ||
                         (d.sy > 0 ? d.py > s.presHi[1] : d.py < s.presLo[1]) ||
-                        (d.sz > 0 ? d.pz > s.presHi[2] : d.pz < s.presLo[2])))
+                        (d.sz > 0 ? d.pz > s.presHi[2] : d.pz < s.presLo[2]) || d.py > colTop))
             break;
     }
     c.cRays++;

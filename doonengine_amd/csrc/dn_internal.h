@@ -50,6 +50,12 @@ struct DNsceneView {
     // chunk units; lo > hi when there is none. Lighting rays stop once beyond
     // it on the side they move toward: no voxel, request or stamp lies there.
     int32_t presLo[3], presHi[3];
+    // Four tables of mapZ x mapX, [q][z][x]: the highest present tile's y over
+    // the columns a ray in column (x, z) can still reach when it moves in
+    // quadrant q (bit 0: x steps up, else x stays or steps down; bit 1: the
+    // same for z), -1 where there is none. A lighting ray that does not move
+    // down stops once above that height (or null: no such rule).
+    const int32_t* colTop;
 };
 
 // Limits of the per-frame tables passed by value to the lighting kernel.

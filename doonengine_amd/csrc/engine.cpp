@@ -136,8 +136,11 @@ struct DNgpuVolume {
     uint32_t* dRequests = nullptr;
     uint32_t* dNumRequests = nullptr;
     uint32_t* dBlockSums = nullptr;
-    bool presDirty = true;             // presLo/presHi must be rebuilt from state
+    bool presDirty = true;             // presLo/presHi and the column tables must be rebuilt from state
     int32_t presLo[3] = {0, 0, 0}, presHi[3] = {0, 0, 0};
+    int32_t* dColTop = nullptr;        // DNsceneView::colTop, 4 x mapZ x mapX
+    size_t colTopCap = 0;              // in entries
+    bool colTopValid = false;          // dColTop holds the tables of the present state
     uint32_t* dWorkCounter = nullptr;  // [0] lighting requests taken, [1] collected requests, [2..3] draw quads,
                                        // [4] direct-ray requests taken
     DNdirectTerm* dDirect = nullptr;   // direct-ray terms per request slot (k_direct)
@@ -267,7 +270,9 @@ void free_tile_buffers(DNgpuVolume* g)
 {
     dfree(g->dFlags); dfree(g->dLastUsed); dfree(g->dChunks); dfree(g->dOccLoaded); dfree(g->dOccPresent);
     dfree(g->dFresh); dfree(g->dLitBits); dfree(g->dVisProp); dfree(g->dRequests); dfree(g->dBlockSums);
-    dfree(g->dReqList); dfree(g->dMarkList); dfree(g->dEditWords);
+    dfree(g->dReqList); dfree(g->dMarkList); dfree(g->dEditWords); dfree(g->dColTop);
+    g->colTopCap = 0;
+    g->colTopValid = false;
     g->editValid.clear();
     g->editSlot.clear();
     g->markValid = false;
@@ -582,6 +587,62 @@ void frame_random(float time, uint32_t nds, uint32_t dbl, DNframeRandom& r)
     for(uint32_t j = 0; j < dbl && j < DN_MAX_BOUNCES; j++) rand_unit_sphere(time + (float)j, r.glossyDir[j]);
 }
 
+// DNsceneView::colTop from the tile states: per column the highest present
+// tile, then for each quadrant its maximum over the columns at or beyond a
+// column in the quadrant's x and z directions. Uploaded through a staging
+// slot on the volume's stream; false (an ERROR reported) leaves the rule off.
+bool upload_column_tops(DNvolume* vol)
+{
+    DNgpuVolume* g = vol->gpu;
+    const uint32_t X = vol->mapSize.x, Y = vol->mapSize.y, Z = vol->mapSize.z;
+    const size_t plane = (size_t)X * Z, n = 4 * plane;
+    if(g->colTopCap < n)
+    {
+        hipStreamSynchronize(g->stream);
+        dfree(g->dColTop);
+        g->colTopCap = 0;
+        if(!dalloc(&g->dColTop, n, "column table")) return false;
+        g->colTopCap = n;
+    }
+    const int s = acquire_stage_slot(g);
+    if(!ensure_stage(g, s, n * sizeof(int32_t))) return false;
+    int32_t* t = (int32_t*)g->hStage[s];
+    int32_t* top = t;                               // quadrant 0 first holds the column tops
+    std::fill(top, top + plane, -1);
+    for(uint32_t z = 0; z < Z; z++)
+        for(uint32_t y = 0; y < Y; y++)
+            for(uint32_t x = 0; x < X; x++)
+                if(g->state[x + (size_t)X * (y + (size_t)Y * z)] != 0) top[(size_t)z * X + x] = (int32_t)y;
+    for(int q = 3; q >= 0; q--)                     // quadrant 0 last: it reads the tops in place
+    {
+        int32_t* o = t + (size_t)q * plane;
+        if(q != 0) std::copy(top, top + plane, o);
+        const bool up_x = (q & 1) != 0, up_z = (q & 2) != 0;
+        for(uint32_t z = 0; z < Z; z++)             // along x within each row
+        {
+            int32_t* row = o + (size_t)z * X;
+            if(up_x)
+            {
+                for(uint32_t x = X - 1; x-- > 0;) row[x] = std::max(row[x], row[x + 1]);
+            }
+            else
+            {
+                for(uint32_t x = 1; x < X; x++) row[x] = std::max(row[x], row[x - 1]);
+            }
+        }
+        for(uint32_t k = 1; k < Z; k++)             // then along z
+        {
+            const uint32_t z = up_z ? Z - 1 - k : k, from = up_z ? z + 1 : z - 1;
+            for(uint32_t x = 0; x < X; x++)
+                o[(size_t)z * X + x] = std::max(o[(size_t)z * X + x], o[(size_t)from * X + x]);
+        }
+    }
+    const bool ok = HIP_OK(hipMemcpyAsync(g->dColTop, t, n * sizeof(int32_t), hipMemcpyHostToDevice, g->stream),
+                           "column table upload");
+    release_stage_slot(g, s);
+    return ok;
+}
+
 DNsceneView scene_view(DNvolume* vol)
 {
     DNgpuVolume* g = vol->gpu;
@@ -616,9 +677,11 @@ DNsceneView scene_view(DNvolume* vol)
                 }
             }
         for(int a = 0; a < 3; a++) { g->presLo[a] = lo[a]; g->presHi[a] = hi[a]; }
+        g->colTopValid = upload_column_tops(vol);
         g->presDirty = false;
     }
     for(int a = 0; a < 3; a++) { s.presLo[a] = g->presLo[a]; s.presHi[a] = g->presHi[a]; }
+    s.colTop = g->colTopValid ? g->dColTop : nullptr;
     return s;
 }
 

@@ -166,6 +166,7 @@ typedef struct ORthread {
     ORcounters cnt;
     int stopOutside;              /* lighting rays: stop beyond the box of present tiles (below) */
     int32_t presLo[3], presHi[3]; /* that box, inclusive; lo > hi when no tile is present */
+    const int32_t* colTop;        /* column tables of present_column_tops, or NULL */
 } ORthread;
 
 /* The GPU's lighting rays end a walk once it is beyond the box of present
@@ -194,6 +195,45 @@ static void present_box(const ORscene* s, int32_t lo[3], int32_t hi[3])
                     if(c[a] > hi[a]) hi[a] = c[a];
                 }
             }
+}
+
+/* The second rule: per column (x, z) the highest present tile, then for each
+ * quadrant q (bit 0: x steps up, else x stays or steps down; bit 1: the same
+ * for z) its maximum over the columns at or beyond (x, z) in those directions:
+ * out[q][z][x], -1 where there is none. A lighting ray whose y does not step
+ * down ends once its y after a step is above the value of the cell it left
+ * (the GPU's DNsceneView::colTop and step_map). out holds 4 * X * Z values;
+ * each quadrant is filled by the recurrence M(x, z) = max(top(x, z),
+ * M(x + dx, z), M(x, z + dz)), visiting the cells from the far corner. */
+void or_present_column_tops(const ORscene* s, int32_t* out)
+{
+    const int32_t X = (int32_t)s->mapSize[0], Y = (int32_t)s->mapSize[1], Z = (int32_t)s->mapSize[2];
+    const size_t plane = (size_t)X * (size_t)Z;
+    int32_t* top = (int32_t*)malloc((plane ? plane : 1) * sizeof(int32_t));
+    for(size_t i = 0; i < plane; i++) top[i] = -1;
+    for(int32_t z = 0; z < Z; z++)
+        for(int32_t y = 0; y < Y; y++)
+            for(int32_t x = 0; x < X; x++)
+                if(s->map[((size_t)x + (size_t)X * ((size_t)y + (size_t)Y * z)) * TILE_WORDS] & 3u)
+                    top[(size_t)z * X + x] = y;
+    for(int q = 0; q < 4; q++)
+    {
+        int32_t* m = out + (size_t)q * plane;
+        const int32_t dx = (q & 1) ? 1 : -1, dz = (q & 2) ? 1 : -1;
+        for(int32_t iz = 0; iz < Z; iz++)
+        {
+            const int32_t z = dz > 0 ? Z - 1 - iz : iz;
+            for(int32_t ix = 0; ix < X; ix++)
+            {
+                const int32_t x = dx > 0 ? X - 1 - ix : ix;
+                int32_t v = top[(size_t)z * X + x];
+                if(x + dx >= 0 && x + dx < X && m[(size_t)z * X + (x + dx)] > v) v = m[(size_t)z * X + (x + dx)];
+                if(z + dz >= 0 && z + dz < Z && m[(size_t)(z + dz) * X + x] > v) v = m[(size_t)(z + dz) * X + x];
+                m[(size_t)z * X + x] = v;
+            }
+        }
+    }
+    free(top);
 }
 
 static void thread_init(ORthread* t, int refraction)
@@ -406,9 +446,16 @@ static int step_map(const ORscene* s, ORthread* t, v3* dir, v3 inv, v3* p, int i
     init_dda(*dir, inv, *p, &d);
     v3 lsd = vs(0);
     int hit = 0;
+    /* the column table of the ray's quadrant, for a ray whose y does not step down */
+    const int32_t* ct = (t->stopOutside && t->colTop && d.step.y >= 0)
+                            ? t->colTop + (size_t)((d.step.x > 0 ? 1 : 0) + (d.step.z > 0 ? 2 : 0)) *
+                                              s->mapSize[0] * s->mapSize[2]
+                            : NULL;
     while(in_map(s, d.pos))
     {
         t->sMap++;
+        /* highest present tile over the columns reachable from this cell */
+        const int32_t reach = ct ? ct[(size_t)d.pos.z * s->mapSize[0] + (size_t)d.pos.x] : INT32_MAX;
         uint32_t idx = map_index_u(s, (uint32_t)d.pos.x, (uint32_t)d.pos.y, (uint32_t)d.pos.z);
         uint32_t flags = s->map[(size_t)idx * TILE_WORDS];
         if((flags & 3u) == 2u)
@@ -441,7 +488,8 @@ static int step_map(const ORscene* s, ORthread* t, v3* dir, v3 inv, v3* p, int i
         ignoreFirst = 0;
         if(t->stopOutside && ((d.step.x > 0 ? d.pos.x > t->presHi[0] : d.pos.x < t->presLo[0]) ||
                               (d.step.y > 0 ? d.pos.y > t->presHi[1] : d.pos.y < t->presLo[1]) ||
-                              (d.step.z > 0 ? d.pos.z > t->presHi[2] : d.pos.z < t->presLo[2])))
+                              (d.step.z > 0 ? d.pos.z > t->presHi[2] : d.pos.z < t->presLo[2]) ||
+                              d.pos.y > reach))
             break;
     }
     t->cnt.calls++;
@@ -935,6 +983,7 @@ typedef struct LRange {
     uint8_t* lit;
     uint32_t* flagsW;
     uint32_t* lastW;
+    const int32_t* colTop;        /* or_present_column_tops of the dispatch-start state */
     ORcounters cnt;
 } LRange;
 
@@ -949,6 +998,7 @@ static void light_range(LRange* j)
     thread_init(&t, 0);
     t.stopOutside = or_stop_outside;
     present_box(s, t.presLo, t.presHi);
+    t.colTop = j->colTop;
     LCtx c;
     c.s = &sc; c.lp = lp; c.t = &t; c.visProp = j->prop;
     c.sunDir = vf(lp->sunDir);
@@ -1050,10 +1100,13 @@ void or_lighting_mt(const ORscene* s, const ORlightParams* lp, const uint32_t* r
     if(numThreads < 1) numThreads = 1;
     if(numThreads > numRequests) numThreads = numRequests ? numRequests : 1;
     LRange* jobs = (LRange*)calloc(numThreads, sizeof(LRange));
+    const size_t plane = (size_t)s->mapSize[0] * s->mapSize[2];
+    int32_t* colTop = (int32_t*)malloc((plane > 0 ? 4 * plane : 1) * sizeof(int32_t));
+    or_present_column_tops(s, colTop);
     for(uint32_t k = 0; k < numThreads; k++)
     {
         LRange* j = &jobs[k];
-        j->s = s; j->lp = lp; j->requests = requests;
+        j->s = s; j->lp = lp; j->requests = requests; j->colTop = colTop;
         j->r0 = (uint32_t)((uint64_t)numRequests * k / numThreads);
         j->r1 = (uint32_t)((uint64_t)numRequests * (k + 1) / numThreads);
         j->voxelsOut = voxelsOut; j->preq = preq; j->voxOf = voxOf;
@@ -1101,6 +1154,7 @@ void or_lighting_mt(const ORscene* s, const ORlightParams* lp, const uint32_t* r
         free(jobs[k].prop); free(jobs[k].lit); free(jobs[k].flagsW); free(jobs[k].lastW);
     }
     free(jobs);
+    free(colTop);
     if(cnt) *cnt = total;
 }
 

@@ -613,10 +613,47 @@ def test_threaded_oracle_equals_single_thread(o, o64):
                 {k: base[5][k] for k in ("calls", "sMap", "sVox", "hits")}
 
 
+def _column_tops_brute(present: np.ndarray) -> np.ndarray:
+    """[q][z][x]: the highest present tile's y over the columns (x', z') with
+    x' >= x when q & 1 else x' <= x, and z' >= z when q & 2 else z' <= z."""
+    Z, Y, X = present.shape
+    ys = np.arange(Y)[None, :, None]
+    top = np.where(present, ys, -1).max(axis=1) if Y else np.full((Z, X), -1)
+    out = np.full((4, Z, X), -1, np.int64)
+    for q in range(4):
+        for z in range(Z):
+            for x in range(X):
+                xs = slice(x, X) if q & 1 else slice(0, x + 1)
+                zs = slice(z, Z) if q & 2 else slice(0, z + 1)
+                out[q, z, x] = top[zs, xs].max()
+    return out
+
+
+@pytest.mark.parametrize("shape,fill", [((5, 3, 7), 0.15), ((1, 4, 1), 0.5), ((4, 2, 3), 0.0), ((6, 5, 4), 0.02)])
+def test_present_column_tops_brute_force(o, o64, shape, fill):
+    """The column tables of the lighting walk's second stop rule, against a
+    brute-force maximum over each cell's reachable columns (present = map
+    flags & 3 != 0, loaded or not)."""
+    Z, Y, X = shape
+    rng = np.random.default_rng(Z * 100 + Y * 10 + X)
+    flags = np.where(rng.random((Z, Y, X)) < fill, rng.integers(1, 4, (Z, Y, X)), 0).astype(np.uint32)
+    map_ = np.zeros((X * Y * Z, 3), np.uint32)
+    map_[:, 0] = flags.reshape(-1)                       # tile index x + X (y + Y z)
+    want = _column_tops_brute(flags != 0)
+    for orc in (o, o64):
+        s = orc.scene((X, Y, Z), map_, np.zeros((X * Y * Z, 24), np.uint32), np.zeros((1, 4), np.uint32),
+                      np.zeros((256, 8), np.uint32))
+        got = np.zeros(4 * X * Z, np.int32)
+        orc.lib.or_present_column_tops(ctypes.byref(s), got.ctypes.data_as(ctypes.c_void_p))
+        assert np.array_equal(got.reshape(4, Z, X), want)
+
+
 def test_stop_outside_present_box_changes_only_counters(o, o64):
     """Lighting rays end once they are beyond the box of present tiles on the
-    side they move toward (the rule of the GPU's lighting walk, which the
-    oracle applies while or_stop_outside is set). With the rule on and off the
+    side they move toward, and rays that do not move down once above every
+    present tile of the columns they can still reach (the rules of the GPU's
+    lighting walk, which the oracle applies while or_stop_outside is set).
+    With the rules on and off the
     lighting pass gives the same stored words, pre-quantisation values, flags
     (visible bits and the requests of unloaded tiles), lastUsed stamps and
     sample counts, and the same rays, voxel trips and hits; only the
