@@ -18,7 +18,8 @@ per second); beside it:
   8 TB/s, and, from the committed rocprofv3 summary when it was taken of the
   same frame sources, its measured HBM traffic and fraction and the VALU
   counters that bound it;
-- `cpu_baseline`: the FP32 oracle on a bounded sample of the frame, on this host;
+- `cpu_baseline`: the FP32 oracle on a bounded sample of the frame, on this host's
+  threads (at most 16);
 - `reference_settings`: the same frame lit with the reference's own lighting
   algorithm (one jittered shadow ray per sample, 15 specular rays, <= 5
   bounces), the like-for-like figure for the llvmpipe baseline;
@@ -90,9 +91,10 @@ def algorithmic_bytes(c: dict) -> float:
             + c.get("litVoxels", 0) * (16.0 + 12.0 + 96.0 / 32.0 + 4.0 / 32.0))
 
 
-def cpu_baseline(vol, sc, budget_s: float) -> dict:
-    """Times the FP32 oracle (single thread) on a bounded prefix of the frame's
-    lighting requests and a band of draw rows, on the exported device state."""
+def cpu_baseline(vol, sc, budget_s: float, threads: int) -> dict:
+    """Times the FP32 oracle on `threads` host threads (request ranges and draw
+    rows split over them) on a bounded prefix of the frame's lighting requests
+    and a band of draw rows, on the exported device state."""
     import numpy as np
 
     import oracle
@@ -118,7 +120,7 @@ def cpu_baseline(vol, sc, budget_s: float) -> dict:
     n, lt, lrays, lit, size = 0, 0.0, 0, 0, 16
     while n < len(req) and lt < 0.7 * budget_s:
         take = min(size, len(req) - n)
-        _, _, _, _, voxof, lc = o.lighting(s, req[n:n + take], **kw)
+        _, _, _, _, voxof, lc = o.lighting(s, req[n:n + take], threads=threads, **kw)
         lt += lc["seconds"]
         lrays += lc["calls"]
         lit += int((voxof != 0xFFFFFFFF).sum())
@@ -127,14 +129,14 @@ def cpu_baseline(vol, sc, budget_s: float) -> dict:
     # Draw: the first rows of the frame (a row band costs about the same per ray).
     view, proj = vol.view_projection(sc.height / sc.width)
     iv, ivp = draw_matrices(view, proj)
-    _, _, dc = o.draw(s, mat_to_np(iv), mat_to_np(ivp), sc.width, 8)
+    _, _, dc = o.draw(s, mat_to_np(iv), mat_to_np(ivp), sc.width, 8, threads=threads)
     rows = int(min(sc.height, max(8, 8 * 0.3 * budget_s / max(dc["seconds"], 1e-3))))
     if rows > 8:
-        _, _, dc = o.draw(s, mat_to_np(iv), mat_to_np(ivp), sc.width, rows)
+        _, _, dc = o.draw(s, mat_to_np(iv), mat_to_np(ivp), sc.width, rows, threads=threads)
     dtime = dc["seconds"]
     rays = lrays + dc["calls"]
-    return {"value": rays / (lt + dtime), "unit": "rays/s", "cores": 1, "kind": "port",
-            "sample": f"FP32 oracle, 1 thread: lighting of the first {n} of {len(req)} requests "
+    return {"value": rays / (lt + dtime), "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": f"FP32 oracle, {threads} host thread(s): lighting of the first {n} of {len(req)} requests "
                       f"({lit} voxels, {lrays} rays, {lt:.1f} s) + draw of {rows} rows x {sc.width} "
                       f"({dc['calls']} rays, {dtime:.1f} s)",
             "lighting_updates_per_s": lit / max(lt, 1e-9)}
@@ -498,6 +500,8 @@ def main() -> int:
     ap.add_argument("--cache-ema-growth", type=float, default=8.0,
                     help="cache parameter average: decay min(ema, 1 - G / step) while the window grows (0: off)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
+                    help="host threads of the CPU baseline (the oracle's request ranges and draw rows)")
     ap.add_argument("--ref-frames", type=int, default=5,
                     help="frame workload: frames timed with the reference's own lighting settings (0: skip)")
     ap.add_argument("--no-summaries", dest="summaries", action="store_false",
@@ -584,7 +588,7 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
-        cpu = cpu_baseline(vol, sc, args.cpu_seconds)
+        cpu = cpu_baseline(vol, sc, args.cpu_seconds, max(1, args.cpu_threads))
 
     # The same frame with the reference's own lighting algorithm
     # (voxelLighting.comp:207-288 at its defaults): one jittered shadow ray per
