@@ -700,13 +700,9 @@ bool ensure_direct(DNgpuVolume* g, uint32_t maxReq)
     return true;
 }
 
-void clear_chunk(DNvolume* vol, size_t index)   // voxel.c:1353-1363
+void clear_chunk_voxels(DNvolume* vol, size_t index)
 {
     DNchunk& c = vol->chunks[index];
-    c.pos = DNivec3{-1, -1, -1};
-    c.updated = false;
-    c.numVoxels = 0;
-    c.numVoxelsGpu = 0;
     for(int x = 0; x < 8; x++)
     for(int y = 0; y < 8; y++)
     for(int z = 0; z < 8; z++)
@@ -714,6 +710,21 @@ void clear_chunk(DNvolume* vol, size_t index)   // voxel.c:1353-1363
         c.voxels[x][y][z].normal = UINT32_MAX;
         c.voxels[x][y][z].albedo = 0;
     }
+}
+
+void clear_chunk_header(DNvolume* vol, size_t index)
+{
+    DNchunk& c = vol->chunks[index];
+    c.pos = DNivec3{-1, -1, -1};
+    c.updated = false;
+    c.numVoxels = 0;
+    c.numVoxelsGpu = 0;
+}
+
+void clear_chunk(DNvolume* vol, size_t index)   // voxel.c:1353-1363
+{
+    clear_chunk_header(vol, index);
+    clear_chunk_voxels(vol, index);
 }
 
 inline uint32_t flat(DNvolume* vol, DNivec3 p)
@@ -972,6 +983,10 @@ extern "C" int DN_add_chunk(DNvolume* vol, DNivec3 pos)
     size_t i = vol->nextChunk;
     do
     {
+        // the search reads one position per 4 KiB chunk, each on a page of its
+        // own: fetch the positions a few slots ahead so the misses overlap
+        const size_t ahead = (i + 8) % vol->chunkCap;
+        __builtin_prefetch(&vol->chunks[ahead].pos);
         if(!DN_in_map_bounds(vol, vol->chunks[i].pos))
         {
             uint32_t mi = flat(vol, pos);
@@ -2362,7 +2377,9 @@ extern "C" bool DNX_shape_spheres(DNvolume* vol, const DNXsphereEdit* edits, uin
                                       g->stream) == 0;
     ok = ok && dn_launch_sphere_edits(ep, U, g->stream) == 0;
     // the events and tile states come back first; the final words of every
-    // touched tile follow while the host replays the events
+    // touched tile follow while the host replays the events (copying them in
+    // quarters, each copied into the CPU chunks once back, measured slower:
+    // 0.88 against 0.82 ms per 256-edit call)
     ok = ok && HIP_OK(hipMemcpyAsync(h + oEv, d + oEv, oGW - oEv, hipMemcpyDeviceToHost, g->stream), "edit readback");
     ok = ok && HIP_OK(hipEventRecord(g->editEvents, g->stream), "edit event");
     ok = ok && dn_launch_edit_gather(g->dEditWords, ep.tileIds, (uint2*)(d + oGW), U, g->stream) == 0;
@@ -2374,7 +2391,22 @@ extern "C" bool DNX_shape_spheres(DNvolume* vol, const DNXsphereEdit* edits, uin
         return false;
     }
 
-    // replay the chunk events in the reference's order
+    // Replay the chunk events in the reference's order. A removal clears the
+    // chunk's 512 voxels (DN_remove_chunk does so at once); most slots freed
+    // here are taken again by a later add of the same call, and every added
+    // tile is written, so the copy-back below overwrites its whole chunk. The
+    // voxels are therefore cleared at the end instead: those of slots still
+    // free then, and on an early return those of every slot freed here.
+    struct PendingClear {
+        DNvolume* vol;
+        std::vector<uint32_t> slots;
+        bool copied;
+        ~PendingClear()
+        {
+            for(uint32_t s : slots)
+                if(!copied || !DN_in_map_bounds(vol, vol->chunks[s].pos)) clear_chunk_voxels(vol, s);
+        }
+    } pending{vol, {}, false};
     const uint8_t* ev = h + oEv;
     for(uint32_t q = 0; q < P; q++)
     {
@@ -2397,7 +2429,15 @@ extern "C" bool DNX_shape_spheres(DNvolume* vol, const DNXsphereEdit* edits, uin
             }
         }
         else
-            DN_remove_chunk(vol, mp);
+        {
+            // DN_remove_chunk with the voxel clearing left to `pending`
+            const uint32_t mi = flat(vol, mp), slot = vol->map[mi].chunkIndex;
+            vol->map[mi].flag = 0;
+            vol->nextChunk = slot;
+            clear_chunk_header(vol, slot);
+            mark_dirty(g, mi);
+            pending.slots.push_back(slot);
+        }
     }
     // final words of the tiles that hold a chunk and were written
     const uint32_t* out = (const uint32_t*)(h + oOut);
@@ -2453,6 +2493,7 @@ extern "C" bool DNX_shape_spheres(DNvolume* vol, const DNXsphereEdit* edits, uin
         copy_range(done, G);
         for(std::thread& t : pool) t.join();
     }
+    pending.copied = true;
     for(uint32_t u = 0; u < U; u++) g->editValid[tileIds[u]] = 1;
     return true;
 }
