@@ -101,7 +101,8 @@ def test_steps_drawn_together_equal_steps_drawn_alone(small_volume):
     """DNX_rc_train_frame draws the training samples of several steps in one
     launch, batch k with the seed of the frame counter plus k: the last step's
     samples of a 3-step frame equal, bit for bit, those of the third of three
-    1-step frames, and differ from the first step's."""
+    1-step frames, and differ from the first step's; a 17-step frame (two
+    launches) ends on the samples of the 17th of 17 1-step frames."""
     vol, _ = small_volume
     vol.mark_visible(1 << 20)
     vol.sync()
@@ -120,6 +121,11 @@ def test_steps_drawn_together_equal_steps_drawn_alone(small_volume):
         assert (sa[:, 12] == 1.0).sum() > 2000                       # rows holding a drawn voxel
         assert np.array_equal(sa.view(np.uint32), sb.view(np.uint32))
         assert not np.array_equal(sa.view(np.uint32), first.view(np.uint32))
+        # more steps than one launch draws (16): a launch of 16 batches, then one of 1
+        a.train_frame(11.0, 17)
+        for _ in range(17):
+            b.train_frame(11.0, 1)
+        assert np.array_equal(a.last_samples().view(np.uint32), b.last_samples().view(np.uint32))
     finally:
         a.close()
         b.close()
