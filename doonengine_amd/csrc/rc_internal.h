@@ -72,7 +72,15 @@ int dn_rc_encode(const DNrcNet& net, const float* samples, uint32_t n, uint16_t*
 // pred: float4 per row (3 outputs, the row's valid flag; 1 without samples).
 int dn_rc_forward(const DNrcNet& net, const uint16_t* x0, const float* samples, uint32_t n, float* pred,
                   hipStream_t stream);
-int dn_rc_train(const DNrcNet& net, const float* samples, uint32_t n, const DNrcActs& a, hipStream_t stream);
+// A second stream for training: the weight gradients run on it beside the
+// table gradients (`fork` and `join` order it after the MLP and before Adam).
+struct DNrcSide {
+    hipStream_t stream;
+    hipEvent_t fork, join;
+};
+// side: null runs every kernel on `stream`.
+int dn_rc_train(const DNrcNet& net, const float* samples, uint32_t n, const DNrcActs& a, hipStream_t stream,
+                const DNrcSide* side = nullptr);
 // Allocation-free prediction: dn_rc_encode + dn_rc_forward with the encoded rows
 // in the fixed per-device scratch array of rcache.hip (at most
 // DNX_RC_NOALLOC_MAX_SAMPLES rows) on device `device`. dn_noalloc_init (called

@@ -29,6 +29,7 @@ struct DNXradianceCache {
     uint32_t adamStep = 0;
     double emaKeep = 1.0;         // product of the average's decays so far (its debiasing)
     uint32_t frame = 0;
+    DNrcSide side = {nullptr, nullptr, nullptr};   // training's second stream (side.stream null: none)
 };
 
 namespace {
@@ -200,6 +201,13 @@ extern "C" void DNX_rc_destroy(DNXradianceCache* rc)
 {
     if(!rc) return;
     if(rc->stream) hipStreamSynchronize(rc->stream);
+    if(rc->side.stream)
+    {
+        hipStreamSynchronize(rc->side.stream);
+        hipStreamDestroy(rc->side.stream);
+        hipEventDestroy(rc->side.fork);
+        hipEventDestroy(rc->side.join);
+    }
     DNrcNet& n = rc->net;
     dfree(n.tab16); dfree(n.tab32); dfree(n.tabGrad); dfree(n.tabM); dfree(n.tabV); dfree(n.tabE); dfree(n.tabE16);
     dfree(n.w16); dfree(n.wT16); dfree(n.w32); dfree(n.wGrad); dfree(n.wM); dfree(n.wV); dfree(n.wE); dfree(n.wE16);
@@ -231,6 +239,16 @@ extern "C" DNXradianceCache* DNX_rc_create(DNvolume* vol, const DNXrcConfig* con
     std::memset(&rc->acts, 0, sizeof(rc->acts));
     rc->vol = vol;
     rc->stream = (hipStream_t)DNX_get_stream(vol);
+    // a second stream for training (default priority); without it training runs on one
+    if(hipStreamCreateWithFlags(&rc->side.stream, hipStreamNonBlocking) != hipSuccess ||
+       hipEventCreateWithFlags(&rc->side.fork, hipEventDisableTiming) != hipSuccess ||
+       hipEventCreateWithFlags(&rc->side.join, hipEventDisableTiming) != hipSuccess)
+    {
+        if(rc->side.stream) hipStreamDestroy(rc->side.stream);
+        if(rc->side.fork) hipEventDestroy(rc->side.fork);
+        if(rc->side.join) hipEventDestroy(rc->side.join);
+        rc->side = DNrcSide{nullptr, nullptr, nullptr};
+    }
     if(hipGetDevice(&rc->device) != hipSuccess) rc->device = 0;
     if(c.extent <= 0.0f)
     {
@@ -308,7 +326,8 @@ extern "C" bool DNX_rc_train_samples(DNXradianceCache* rc, const float* samples,
 {
     if(!rc || !samples || n == 0 || (n & 15u)) return false;
     if(!ensure_train(rc, n)) return false;
-    if(dn_rc_train(rc->net, samples, n, rc->acts, rc->stream) != 0) return false;
+    if(dn_rc_train(rc->net, samples, n, rc->acts, rc->stream, rc->side.stream ? &rc->side : nullptr) != 0)
+        return false;
     return !step || adam_step(rc);
 }
 
@@ -367,7 +386,9 @@ extern "C" bool DNX_rc_train_frame(DNXradianceCache* rc, float time, int steps)
             const float* smp = rc->dSamples + (size_t)k * n * RC_SAMPLE;
             rc->lastSamplesPtr = smp;
             rc->lastSamples = n;
-            if(dn_rc_train(rc->net, smp, n, rc->acts, rc->stream) != 0 || !adam_step(rc)) return false;
+            if(dn_rc_train(rc->net, smp, n, rc->acts, rc->stream, rc->side.stream ? &rc->side : nullptr) != 0 ||
+               !adam_step(rc))
+                return false;
         }
     }
     return true;

@@ -704,7 +704,8 @@ int dn_rc_forward(const DNrcNet& net, const uint16_t* x0, const float* samples, 
     return (int)hipGetLastError();
 }
 
-int dn_rc_train(const DNrcNet& net, const float* samples, uint32_t n, const DNrcActs& a, hipStream_t stream)
+int dn_rc_train(const DNrcNet& net, const float* samples, uint32_t n, const DNrcActs& a, hipStream_t stream,
+                const DNrcSide* side)
 {
     if(n == 0 || (n & 15u)) return n ? (int)hipErrorInvalidValue : 0;
     const uint32_t tiles = n / 16u;
@@ -713,12 +714,26 @@ int dn_rc_train(const DNrcNet& net, const float* samples, uint32_t n, const DNrc
     // 1024 blocks, 98 at 256 and 82 at 768
     hipLaunchKernelGGL(k_rc_mlp<true>, dim3(blocks_for((size_t)tiles * 64u, 512)), dim3(256), 0, stream, net,
                        (const uint16_t*)a.x0, samples, n, 1.0f / (float)n, (float*)nullptr, a);
+    // The weight gradients (MFMA and LDS bound) and the table gradients (bound
+    // by their atomics) read the same activations and write disjoint buffers:
+    // with a side stream they run side by side.
+    // one block per CU (47 us per 2^17 samples; 128 blocks 71 us, 512 blocks 50 us)
+    const uint32_t wb = tiles < 256u ? tiles : 256u;
+    hipStream_t ws = stream;
+    if(side)
+    {
+        if(hipEventRecord(side->fork, stream) != hipSuccess ||
+           hipStreamWaitEvent(side->stream, side->fork, 0) != hipSuccess)
+            return (int)hipErrorLaunchFailure;
+        ws = side->stream;
+    }
+    hipLaunchKernelGGL(k_rc_wgrad, dim3(wb), dim3(256), 0, ws, net, tiles, a);
     // one wave per 16 samples and level
     hipLaunchKernelGGL(k_rc_table_grad, dim3(blocks_for((size_t)RC_LEVELS * ((n + 15u) / 16u) * 64u)), dim3(256), 0,
                        stream, net, samples, n, (const float*)a.denc);
-    // one block per CU (47 us per 2^17 samples; 128 blocks 71 us, 512 blocks 50 us)
-    const uint32_t wb = tiles < 256u ? tiles : 256u;
-    hipLaunchKernelGGL(k_rc_wgrad, dim3(wb), dim3(256), 0, stream, net, tiles, a);
+    if(side && (hipEventRecord(side->join, side->stream) != hipSuccess ||
+                hipStreamWaitEvent(stream, side->join, 0) != hipSuccess))
+        return (int)hipErrorLaunchFailure;
     return (int)hipGetLastError();
 }
 
